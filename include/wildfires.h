@@ -1,0 +1,368 @@
+/*
+ * wildfires.h -- C-ABI of the MI355X-native time-step library (libwildfires_amd.so).
+ *
+ * Two layers are exported:
+ *
+ *  1. The reference's own solver entry points, with the same names, argument
+ *     meaning, parameter-file keys, output files and exit behaviour, so that a
+ *     driver written against the reference's C API (src/cu/main.cu:3-94) links
+ *     against this library unchanged.  The Parameters structure keeps the
+ *     reference's member names, types and order (include/c/structures.h:108-259)
+ *     because the driver passes it by value to every entry point.
+ *
+ *  2. wf_* entry points: the hot path split into the pieces the reference calls
+ *     inside solve_PDE (src/cu/solver.cu:95-298), over a solver handle that owns
+ *     all device memory.  Host buffers at this boundary use the reference layout:
+ *     state vector y = [u | v | w | T | Y], scalar fields indexed
+ *     IDX(i,j,k) = k + Nz*(j + Ny*i) (include/cu/utils.cuh:17-19), Y compact at
+ *     Nz_Y_max levels.  Every wf_* function returns 0 on success or a negative
+ *     WF_E* status; wf_last_error() returns a message for the last failure.
+ *
+ * Plain C types only; no C++ or torch types cross this boundary.
+ */
+#ifndef WILDFIRES_AMD_H
+#define WILDFIRES_AMD_H
+
+#include <stddef.h>
+#include <stdio.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- reference data structures (include/c/structures.h:21-94) ---------------- */
+
+typedef struct _field_indexes {
+    int u;
+    int v;
+    int w;
+    int T;
+    int Y;
+} FieldIndexes;
+
+typedef struct _pressure_indexes {
+    int ux;
+    int vy;
+    int wz;
+} PressureIndexes;
+
+typedef struct _turbulence_indexes {
+    int rho;
+    int Tx;
+    int Ty;
+    int Tz;
+    int Txx;
+    int Tyy;
+    int Tzz;
+    int mu_sgs;
+    int S_11;
+    int S_12;
+    int S_13;
+    int S_21;
+    int S_22;
+    int S_23;
+    int S_31;
+    int S_32;
+    int S_33;
+    int div_U;
+    int u_tau_wall;
+} TurbulenceIndexes;
+
+typedef struct _log_files {
+    FILE *parameters;
+    FILE *log;
+} LogFiles;
+
+/* Parameters: include/c/structures.h:108-259 (same members, same order). */
+typedef struct _parameters {
+    int Nx;
+    int Ny;
+    int Nz;
+    int Nt;
+    int NT;
+    double x_min;
+    double x_max;
+    double y_min;
+    double y_max;
+    double z_min;
+    double z_max;
+    double t_min;
+    double t_max;
+    double dx;
+    double dy;
+    double dz;
+    double dt;
+    double *x;
+    double *y;
+    double *z;
+    double *t;
+    double *r;
+    double *s;
+    double *kx;
+    double *ky;
+    double c_p;
+    double c_v;
+    double rho_inf;
+    double mu;
+    double k;
+    double alpha;
+    double delta;
+    double nu;
+    double T_inf;
+    double g;
+    double C_s;
+    double Pr;
+    double C_d;
+    double alpha_s;
+    double sigma_s;
+    double T_pc;
+    double H_C;
+    double A;
+    double E_A;
+    double T_act;
+    double h_c;
+    double Y_D;
+    double Y_f;
+    double T_0;
+    double T_source;
+    double Y_h;
+    int k_Y_h;
+    int Nz_Y_max;
+    int *Nz_Y;
+    double T_min;
+    double T_max;
+    double Y_min;
+    double Y_max;
+    char U0_type[32];
+    double u_z0;
+    double d;
+    double u_ast;
+    double u_r;
+    double z_r;
+    double alpha_u;
+    char T0_shape[32];
+    double T0_x_start;
+    double T0_x_end;
+    double T0_x_center;
+    double T0_length;
+    double T0_y_start;
+    double T0_y_end;
+    double T0_y_center;
+    double T0_width;
+    double T0_z_start;
+    double T0_z_end;
+    double T0_z_center;
+    double T0_height;
+    double Y0_x_start;
+    double Y0_x_end;
+    double Y0_y_start;
+    double Y0_y_end;
+    double Y0_xa;
+    double Y0_ya;
+    char topo_shape[32];
+    double *topography;
+    double hill_center_x;
+    double hill_center_y;
+    double hill_length;
+    double hill_width;
+    double hill_height;
+    char T_source_shape[32];
+    double t_source_start;
+    double t_source_end;
+    double TS_x_start;
+    double TS_x_end;
+    double TS_x_center;
+    double TS_length;
+    double TS_y_start;
+    double TS_y_end;
+    double TS_y_center;
+    double TS_width;
+    double TS_z_start;
+    double TS_z_end;
+    double TS_z_center;
+    double TS_height;
+    int *cut_nodes;
+    double *z_ibm;
+    double p_top;
+    double u_dead_nodes;
+    double v_dead_nodes;
+    double w_dead_nodes;
+    double T_dead_nodes;
+    double Y_dead_nodes;
+    int variable_density;
+    double pressure_solver_tol;
+    int pressure_solver_iter;
+    int pressure_solver_log;
+    char method[32];
+    char save_path[64];
+    char input_path[64];
+    char sim_id[32];
+    int n_threads;
+    int z_domain;
+    double k_nu_grid;
+    int k_transition;
+    double z_transition;
+    int velocity_correction_fd;
+    int time_step_log;
+    int temperature_convection;
+    int sutherland_viscosity;
+    int sutherland_conductivity;
+    size_t threads_per_block;
+    size_t number_of_blocks;
+    FieldIndexes field_indexes;
+    TurbulenceIndexes turbulence_indexes;
+    PressureIndexes pressure_indexes;
+    LogFiles log_files;
+} Parameters;
+
+/* Physical constants of the reference (include/c/constants.h:17-20). */
+#define WF_SIGMA 5.670374419e-8 /* Stefan-Boltzmann constant */
+#define WF_A_VD 26.0            /* Van Driest damping constant */
+
+/* ---- layer 1: reference entry points ------------------------------------------ */
+
+/* src/cu/parameters.cu:13-554.  Creates data/output/<YYYYMMDDHHMMSS>/ under the
+ * current directory (or under $WILDFIRES_OUTPUT_ROOT when set), opens
+ * parameters.txt and log.txt there, parses the `key = value` file with the
+ * reference's prefix matching, builds grids, wavenumbers, topography, the IBM
+ * mask and the state-vector index tables. */
+Parameters read_parameters_file(const char *file_path);
+/* src/cu/parameters.cu:556-571 */
+void free_parameters(Parameters parameters);
+
+/* src/cu/functions.cu:187-323 -- analytic initial conditions. */
+void initial_conditions(double *u, double *v, double *w, double *T, double *Y, double *p, Parameters parameters);
+/* src/cu/input.cu:3-58 -- initial conditions from <input_path>{u,v,w,T,Y,p}.bin;
+ * exits with the reference's messages on a missing or short file. */
+void initial_conditions_input(double *u, double *v, double *w, double *T, double *Y, double *p, Parameters parameters);
+/* src/cu/solver.cu:41-57 -- pack u,v,w,T,Y into y_0. */
+void create_y_0(double *u, double *v, double *w, double *T, double *Y, double *y_0, Parameters parameters);
+
+/* src/cu/output.cu:3-98 -- raw little-endian float64 snapshots (int32 for Nz_Y). */
+void save_data(double *data, double *p, int n, Parameters parameters);
+void save_domain(Parameters parameters);
+void save_topography(Parameters parameters);
+
+/* src/cu/logs.cu:3-154 */
+void log_parameters_report(Parameters parameters, int to_file);
+void log_parameters(Parameters parameters);
+void log_message(Parameters parameters, const char *message);
+void log_timestep(Parameters parameters, int n, double t_n, double step_time, double CFL,
+                  double T_min, double T_max, double Y_min, double Y_max);
+
+/* src/cu/functions.cu:82-185 -- CFL, fuel-box T/Y min/max, NaN check (exits). */
+void timestep_reports(double *y_n, double *CFL, double *Y_min, double *Y_max, double *T_min,
+                      double *T_max, int n, Parameters parameters);
+
+/* src/cu/solver.cu:95-298 -- the whole time loop.  y_n and p are HOST arrays
+ * (reference layout), as in the reference's CPU backends (src/c/solver.c:79);
+ * the library owns every device buffer.  On return y_n and p hold the state at
+ * t_max.  Checkpoints are written every NT steps and at step Nt. */
+void solve_PDE(double *y_n, double *p, Parameters parameters);
+
+/* Grid and terrain helpers (src/cu/utils.cu:32-51, src/cu/functions.cu:18-80,
+ * src/cu/ibm.cu:3-95). */
+void fft_freq(double *f, int N, double d);
+void equispaced_domain(double *z, int Nz, double z_min, double dz);
+void non_equispaced_domain(double *z, int Nz, double z_min, double z_max, double k);
+void transition_domain(double *z, int Nz, double z_min, double z_max, double z_t, int k_t);
+void flat_terrain(Parameters *parameters);
+void simple_hill(Parameters *parameters);
+void ibm_parameters(Parameters *parameters);
+double power_law(double z, double u_r, double z_r, double alpha_u);
+double gaussian(double x, double y, double z, double x_0, double y_0, double z_0, double sx, double sy, double sz);
+double parallelepiped(double x, double y, double z, double x_min, double x_max, double y_min,
+                      double y_max, double z_min, double z_max);
+
+/* ---- layer 2: wf_* hot-path entry points -------------------------------------- */
+
+#define WF_OK 0
+#define WF_E_ARG (-1)     /* bad argument or unsupported configuration */
+#define WF_E_DEVICE (-2)  /* no usable gfx950 device / HIP runtime error */
+#define WF_E_ALLOC (-3)   /* device allocation failed */
+#define WF_E_NAN (-4)     /* non-finite value detected */
+#define WF_E_METHOD (-5)  /* unknown time-integration method */
+
+typedef struct wf_solver wf_solver;
+
+/* Message of the last failing wf_* call on this thread. */
+const char *wf_last_error(void);
+/* Library version string. */
+const char *wf_version(void);
+
+/* Parameters on the heap, for FFI callers that cannot hold the struct by value.
+ * wf_parameters_read is read_parameters_file; wf_parameters_free is
+ * free_parameters followed by free(). */
+Parameters *wf_parameters_read(const char *file_path);
+void wf_parameters_free(Parameters *parameters);
+size_t wf_parameters_sizeof(void);
+/* State-vector length S = 4*Nx*Ny*Nz + Nx*Ny*Nz_Y_max (src/cu/solver.cu:102). */
+long wf_state_size(const Parameters *parameters);
+
+/* Setup part of solve_PDE (src/cu/solver.cu:130-173): device workspace, grids,
+ * IBM mask, pressure coefficients.  Selects device 0. */
+int wf_solver_create(const Parameters *parameters, wf_solver **out);
+int wf_solver_destroy(wf_solver *solver);
+
+/* Host <-> device state transfer (reference layout on the host side). */
+int wf_solver_set_state(wf_solver *solver, const double *y_n, const double *p);
+int wf_solver_get_state(wf_solver *solver, double *y_n, double *p);
+
+/* n_steps iterations of the body of the time loop (src/cu/solver.cu:178-265)
+ * without file I/O or logging: RK stages, pressure solve, projection, boundary
+ * conditions, bounds, temperature source, y_n <- y_np1.  step0 is the index n
+ * of the first step taken (time t[step0-1] -> t[step0]).  Runs on the device
+ * state; synchronises before returning. */
+int wf_solver_step(wf_solver *solver, int step0, int n_steps);
+
+/* The three calls below are checking entry points: they use the solver's
+ * device buffers as workspace and leave its state undefined; call
+ * wf_solver_set_state before stepping again. */
+
+/* One evaluation of Phi (src/cu/pde.cu:910-917): RHS + SGS turbulence + boundary
+ * conditions, R_old -> R_new, both host vectors of length S. */
+int wf_solver_phi(wf_solver *solver, const double *R_old, double *R_new);
+
+/* Pressure solve of one step (src/cu/pressure.cu:506-606): iterative when
+ * variable_density == 1.  y_np1, y_n host vectors of length S; p host field of
+ * length Nx*Ny*Nz, updated in place.  error/iterations may be NULL. */
+int wf_solver_pressure(wf_solver *solver, const double *y_np1, const double *y_n, double *p,
+                       double *error, int *iterations);
+
+/* Step-end operators (src/cu/solver.cu:204-223): velocity correction, boundary
+ * conditions, bounds and temperature source at step n, applied to y_np1 in place. */
+int wf_solver_project(wf_solver *solver, double *y_np1, const double *y_n, const double *p, int n);
+
+/* Counters since creation: pressure solves summed over steps, steps taken, and
+ * the last pressure-iteration error. */
+int wf_solver_stats(const wf_solver *solver, long *pressure_iterations, long *steps, double *last_error);
+/* Error and reported iteration count of the last pressure solve, as the
+ * reference logs them (src/cu/pressure.cu:597-602). */
+int wf_solver_last_pressure(const wf_solver *solver, double *error, int *iterations);
+
+/* Kernel classes timed by the profiler. */
+#define WF_K_DERIVED 0  /* per-stage derived-field pass */
+#define WF_K_STAGE 1    /* fused RHS + SGS + boundary + RK-combination pass */
+#define WF_K_PRHS 2     /* Poisson right-hand side + forward x-transform */
+#define WF_K_PFY 3      /* y-transforms (forward and inverse) */
+#define WF_K_THOMAS 4   /* tridiagonal solves */
+#define WF_K_PINV 5     /* inverse x-transform + pressure update + norm */
+#define WF_K_PROJECT 6  /* projection + boundary conditions + bounds */
+#define WF_K_OTHER 7    /* small kernels (top duplicates) */
+#define WF_K_COUNT 8
+
+/* Enables (1) or disables (0) per-kernel timing of wf_solver_step: every launch
+ * is bracketed by HIP events on the solver's stream and accumulated per class.
+ * Enabling resets the totals. */
+int wf_solver_profile(wf_solver *solver, int enable);
+/* Accumulated device time (ms) and launch count of one kernel class. */
+int wf_solver_kernel_times(const wf_solver *solver, int kind, double *total_ms, long *launches);
+
+/* Device synchronisation for callers that time whole steps. */
+int wf_device_synchronize(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* WILDFIRES_AMD_H */

@@ -1,0 +1,1067 @@
+/*
+ * wf_oracle.c -- FP64 C/OpenMP oracle of the wildfire time step.
+ *
+ * TEST INFRASTRUCTURE ONLY (see wf_oracle.h).  Every function restates the
+ * reference CUDA code in plain C, cell by cell, in the reference's loop order
+ * and the reference's floating-point operation order, with the per-cell
+ * temporaries private to each OpenMP thread.  Decisions where the reference is
+ * undefined, racy or launch-configuration dependent are marked [Qn]; the list
+ * is kept in the repository's design notes.
+ *
+ * Build: gcc -O2 -fopenmp -fPIC -ffp-contract=off (no FMA contraction, so the
+ * operation order below is the one executed).
+ */
+#include "wf_oracle.h"
+
+#include <math.h>
+#include <omp.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define IDX(i, j, k, Nx, Ny, Nz) ((long)(k) + (long)(Nz) * ((long)(j) + (long)(Ny) * (long)(i)))
+#define FIDX(i, j, k, Nx, Ny) ((long)(j) + (long)(Ny) * (long)(i) + (long)(Nx) * (long)(Ny) * (long)(k))
+#define OMAX(a, b) (((a) > (b)) ? (a) : (b))
+#define OMIN(a, b) (((a) < (b)) ? (a) : (b))
+
+#define O_SIGMA 5.670374419e-8
+#define O_A_VD 26.0
+#ifndef M_PI
+#define M_PI 3.14159265358979323846
+#endif
+
+struct oracle_ctx {
+    Parameters P;      /* shallow copy: grids and masks owned by the caller */
+    long N;            /* Nx*Ny*Nz */
+    long S;            /* state length */
+    long M;            /* (Nx-1)*(Ny-1)*(Nz-1) */
+    double *k;         /* 4 stage derivatives */
+    double *F;         /* stage input */
+    double *y_np1;
+    double *R_turb;    /* 18*N + Nx*Ny */
+    double *f_hat;     /* complex M, interleaved */
+    double *p_top_hat; /* complex (Nx-1)*(Ny-1) */
+    double *p_tmp;     /* N */
+    double *T_source;  /* N */
+    int have_T_source;
+    long solves;
+};
+
+int oracle_threads(void) { return omp_get_max_threads(); }
+
+/* ------------------------------------------------------------------ IBM ---- */
+
+/* src/cu/ibm.cu:57-95 */
+void oracle_ibm(const Parameters *P, const double *topography, int *cut_nodes, int *Nz_Y, double *z_ibm,
+                int *Nz_Y_max) {
+    int Nx = P->Nx, Ny = P->Ny, Nz = P->Nz;
+    const double *z = P->z;
+    int m = 0;
+    for (int i = 0; i < Nx; i++) {
+        for (int j = 0; j < Ny; j++) {
+            double topo = topography[IDX(i, j, 0, Nx, Ny, 1)];
+            cut_nodes[IDX(i, j, 0, Nx, Ny, 1)] = 0;
+            Nz_Y[IDX(i, j, 0, Nx, Ny, 1)] = 0;
+            for (int k = 0; k < Nz; k++) {
+                double dz;
+                if (k < Nz - 1)
+                    dz = z[k + 1] - z[k];
+                else
+                    dz = z[k] - z[k - 1];
+                if ((topo - dz) <= z[k] && z[k] <= topo) cut_nodes[IDX(i, j, 0, Nx, Ny, 1)] = k;
+                z_ibm[IDX(i, j, k, Nx, Ny, Nz)] = OMAX(z[k] - topo, 0.0);
+                if (z[k] <= topo + P->Y_h) Nz_Y[IDX(i, j, 0, Nx, Ny, 1)] = k + 1;
+            }
+            m = OMAX(m, Nz_Y[IDX(i, j, 0, Nx, Ny, 1)]);
+        }
+    }
+    *Nz_Y_max = m;
+}
+
+/* -------------------------------------------------- finite differences ---- */
+
+/* Second-order one-sided first derivative from nodes k, k+1, k+2 with spacings
+ * h0 = z[k+1]-z[k], h1 = z[k+2]-z[k+1] (src/cu/pde.cu:363-365). */
+static inline double d1_fwd(double f0, double f1, double f2, double h0, double h1) {
+    return -(2 * h0 + h1) * f0 / (h0 * (h0 + h1)) + (h0 + h1) * f1 / (h0 * h1) - h0 * f2 / (h1 * (h0 + h1));
+}
+
+/* Second-order one-sided first derivative from nodes k-2, k-1, k with spacings
+ * hm1 = z[k]-z[k-1], hm2 = z[k-1]-z[k-2] (src/cu/pde.cu:383-385). */
+static inline double d1_bwd(double fm2, double fm1, double f0, double hm2, double hm1) {
+    return hm1 * fm2 / (hm2 * (hm2 + hm1)) - (hm2 + hm1) * fm1 / (hm2 * hm1) + (hm2 + 2 * hm1) * f0 / (hm1 * (hm2 + hm1));
+}
+
+/* Four-point second derivative at the bottom node (src/cu/pde.cu:497-500). */
+static inline double d2_fwd(double f0, double f1, double f2, double f3, double h0, double h1, double h2) {
+    return (6 * h0 + 4 * h1 + 2 * h2) * f0 / (h0 * (h0 + h1) * (h0 + h1 + h2)) -
+           (4 * h0 + 4 * h1 + 2 * h2) * f1 / (h0 * h1 * (h1 + h2)) +
+           (4 * h0 + 2 * h1 + 2 * h2) * f2 / (h1 * h2 * (h0 + h1)) -
+           (4 * h0 + 2 * h1) * f3 / (h2 * (h1 + h2) * (h0 + h1 + h2));
+}
+
+/* Four-point second derivative at the top node (src/cu/pde.cu:539-542). */
+static inline double d2_bwd(double fm3, double fm2, double fm1, double f0, double hm3, double hm2, double hm1) {
+    return -(2 * hm2 + 4 * hm1) * fm3 / (hm3 * (hm3 + hm2) * (hm3 + hm2 + hm1)) +
+           (2 * hm3 + 2 * hm2 + 4 * hm1) * fm2 / (hm3 * hm2 * (hm2 + hm1)) -
+           (2 * hm3 + 4 * hm2 + 4 * hm1) * fm1 / (hm2 * hm1 * (hm3 + hm2)) +
+           (2 * hm3 + 4 * hm2 + 6 * hm1) * f0 / (hm1 * (hm2 + hm1) * (hm3 + hm2 + hm1));
+}
+
+/* Interior second derivative on a non-uniform grid (src/cu/pde.cu:572). */
+static inline double d2_mid(double fm1, double f0, double fp1, double hm1, double h0) {
+    return 2 * fm1 / (hm1 * (hm1 + h0)) - 2 * f0 / (hm1 * h0) + 2 * fp1 / (h0 * (hm1 + h0));
+}
+
+/* ------------------------------------------------------------------ RHS ---- */
+
+/* Per-cell right-hand side (src/cu/pde.cu:139-787).  [Q1] the loop-carried h_c
+ * of src/cu/pde.cu:698 is evaluated per cell from the parameter value;
+ * [Q2] the integer 2/3 of src/cu/pde.cu:645 makes the trace term of |S| zero;
+ * [Q12] pow(T,2), pow(T,3) and pow(.,2.0) are taken as products. */
+void oracle_rhs(const Parameters *P, const double *R_old, double *R_new, double *R_turb) {
+    const int Nx = P->Nx, Ny = P->Ny, Nz = P->Nz, Nzm = P->Nz_Y_max;
+    const long size = (long)Nx * Ny * Nz;
+    const double dx = P->dx, dy = P->dy;
+    const double *z = P->z;
+    const int *Nz_Y = P->Nz_Y;
+    const double Y_f = P->Y_f, H_C = P->H_C, A = P->A, T_act = P->T_act, T_pc = P->T_pc;
+    const double h_c0 = P->h_c, alpha_s = P->alpha_s, sigma_s = P->sigma_s, T_inf = P->T_inf;
+    const double c_p = P->c_p, rho_inf = P->rho_inf, C_d = P->C_d, g = P->g, delta = P->delta, C_s = P->C_s;
+    const int conv = P->temperature_convection;
+    const int suth_mu = P->sutherland_viscosity, suth_k = P->sutherland_conductivity;
+    const int ui = P->field_indexes.u, vi = P->field_indexes.v, wi = P->field_indexes.w;
+    const int Ti = P->field_indexes.T, Yi = P->field_indexes.Y;
+    const TurbulenceIndexes ti = P->turbulence_indexes;
+
+#pragma omp parallel for schedule(static)
+    for (long ijk = 0; ijk < size; ijk++) {
+        int i = (int)(ijk / ((long)Ny * Nz));
+        int j = (int)((ijk % ((long)Ny * Nz)) / Nz);
+        int k = (int)(ijk % Nz);
+        int im1 = (i - 1 + Nx - 1) % (Nx - 1), im2 = (i - 2 + Nx - 1) % (Nx - 1);
+        int jm1 = (j - 1 + Ny - 1) % (Ny - 1), jm2 = (j - 2 + Ny - 1) % (Ny - 1);
+        int ip1 = (i + 1) % (Nx - 1), ip2 = (i + 2) % (Nx - 1);
+        int jp1 = (j + 1) % (Ny - 1), jp2 = (j + 2) % (Ny - 1);
+        double mu = P->mu, kT = P->k, dk = 0.0;
+        /* centre and x,y neighbours of u, v, w, T */
+        double c[4], xm1[4], xp1[4], xm2[4], xp2[4], ym1[4], yp1[4], ym2[4], yp2[4];
+        double zm1[4] = {0, 0, 0, 0}, zp1[4] = {0, 0, 0, 0}, zm2[4] = {0, 0, 0, 0}, zp2[4] = {0, 0, 0, 0};
+        double zm3[4] = {0, 0, 0, 0}, zp3[4] = {0, 0, 0, 0};
+        const int off[4] = {ui, vi, wi, Ti};
+        for (int f = 0; f < 4; f++) {
+            const double *F = R_old + off[f];
+            c[f] = F[IDX(i, j, k, Nx, Ny, Nz)];
+            xm1[f] = F[IDX(im1, j, k, Nx, Ny, Nz)];
+            xp1[f] = F[IDX(ip1, j, k, Nx, Ny, Nz)];
+            xm2[f] = F[IDX(im2, j, k, Nx, Ny, Nz)];
+            xp2[f] = F[IDX(ip2, j, k, Nx, Ny, Nz)];
+            ym1[f] = F[IDX(i, jm1, k, Nx, Ny, Nz)];
+            yp1[f] = F[IDX(i, jp1, k, Nx, Ny, Nz)];
+            ym2[f] = F[IDX(i, jm2, k, Nx, Ny, Nz)];
+            yp2[f] = F[IDX(i, jp2, k, Nx, Ny, Nz)];
+            if (k > 0) zm1[f] = F[IDX(i, j, k - 1, Nx, Ny, Nz)];
+            if (k < Nz - 1) zp1[f] = F[IDX(i, j, k + 1, Nx, Ny, Nz)];
+            if (k > 1) zm2[f] = F[IDX(i, j, k - 2, Nx, Ny, Nz)];
+            if (k < Nz - 2) zp2[f] = F[IDX(i, j, k + 2, Nx, Ny, Nz)];
+            if (k > 2) zm3[f] = F[IDX(i, j, k - 3, Nx, Ny, Nz)];
+            if (k < Nz - 3) zp3[f] = F[IDX(i, j, k + 3, Nx, Ny, Nz)];
+        }
+        double u_ijk = c[0], v_ijk = c[1], w_ijk = c[2], T_ijk = c[3];
+        double rho_ijk = T_inf * rho_inf / T_ijk;
+        double Y_ijk = 0.0;
+        /* upwind one-sided differences (src/cu/pde.cu:332-437) */
+        double u_plu = OMAX(u_ijk, 0.0), u_min = OMIN(u_ijk, 0.0);
+        double v_plu = OMAX(v_ijk, 0.0), v_min = OMIN(v_ijk, 0.0);
+        double w_plu = OMAX(w_ijk, 0.0), w_min = OMIN(w_ijk, 0.0);
+        double d_im[4], d_ip[4], d_jm[4], d_jp[4], d_km[4], d_kp[4];
+        for (int f = 0; f < 4; f++) {
+            d_im[f] = (3 * c[f] - 4 * xm1[f] + xm2[f]) / (2 * dx);
+            d_ip[f] = (-3 * c[f] + 4 * xp1[f] - xp2[f]) / (2 * dx);
+            d_jm[f] = (3 * c[f] - 4 * ym1[f] + ym2[f]) / (2 * dy);
+            d_jp[f] = (-3 * c[f] + 4 * yp1[f] - yp2[f]) / (2 * dy);
+            if (k <= 1) {
+                double h0 = z[k + 1] - z[k], h1 = z[k + 2] - z[k + 1];
+                d_km[f] = d1_fwd(c[f], zp1[f], zp2[f], h0, h1);
+            } else {
+                double hm1 = z[k] - z[k - 1], hm2 = z[k - 1] - z[k - 2];
+                d_km[f] = d1_bwd(zm2[f], zm1[f], c[f], hm2, hm1);
+            }
+            if (k >= Nz - 2) {
+                double hm1 = z[k] - z[k - 1], hm2 = z[k - 1] - z[k - 2];
+                d_kp[f] = d1_bwd(zm2[f], zm1[f], c[f], hm2, hm1);
+            } else {
+                double h0 = z[k + 1] - z[k], h1 = z[k + 2] - z[k + 1];
+                d_kp[f] = d1_fwd(c[f], zp1[f], zp2[f], h0, h1);
+            }
+        }
+        double uux = u_plu * d_im[0] + u_min * d_ip[0];
+        double vuy = v_plu * d_jm[0] + v_min * d_jp[0];
+        double wuz = w_plu * d_km[0] + w_min * d_kp[0];
+        double uvx = u_plu * d_im[1] + u_min * d_ip[1];
+        double vvy = v_plu * d_jm[1] + v_min * d_jp[1];
+        double wvz = w_plu * d_km[1] + w_min * d_kp[1];
+        double uwx = u_plu * d_im[2] + u_min * d_ip[2];
+        double vwy = v_plu * d_jm[2] + v_min * d_jp[2];
+        double wwz = w_plu * d_km[2] + w_min * d_kp[2];
+        double uTx = u_plu * d_im[3] + u_min * d_ip[3];
+        double vTy = v_plu * d_jm[3] + v_min * d_jp[3];
+        double wTz = w_plu * d_km[3] + w_min * d_kp[3];
+        /* central differences (src/cu/pde.cu:453-576) */
+        double ux = (xp1[0] - xm1[0]) / (2 * dx), uy = (yp1[0] - ym1[0]) / (2 * dy);
+        double vx = (xp1[1] - xm1[1]) / (2 * dx), vy = (yp1[1] - ym1[1]) / (2 * dy);
+        double wx = (xp1[2] - xm1[2]) / (2 * dx), wy = (yp1[2] - ym1[2]) / (2 * dy);
+        double Tx = (xp1[3] - xm1[3]) / (2.0 * dx), Ty = (yp1[3] - ym1[3]) / (2.0 * dy);
+        double dxx[4], dyy[4], dz1[4], dzz[4];
+        for (int f = 0; f < 4; f++) {
+            dxx[f] = (xp1[f] - 2.0 * c[f] + xm1[f]) / (dx * dx);
+            dyy[f] = (yp1[f] - 2.0 * c[f] + ym1[f]) / (dy * dy);
+        }
+        if (k == 0) {
+            double h0 = z[k + 1] - z[k], h1 = z[k + 2] - z[k + 1], h2 = z[k + 3] - z[k + 2];
+            for (int f = 0; f < 4; f++) {
+                dz1[f] = d1_fwd(c[f], zp1[f], zp2[f], h0, h1);
+                dzz[f] = d2_fwd(c[f], zp1[f], zp2[f], zp3[f], h0, h1, h2);
+            }
+        } else if (k == Nz - 1) {
+            double hm1 = z[k] - z[k - 1], hm2 = z[k - 1] - z[k - 2], hm3 = z[k - 2] - z[k - 3];
+            for (int f = 0; f < 4; f++) {
+                dz1[f] = d1_bwd(zm2[f], zm1[f], c[f], hm2, hm1);
+                dzz[f] = d2_bwd(zm3[f], zm2[f], zm1[f], c[f], hm3, hm2, hm1);
+            }
+        } else {
+            double hm1 = z[k] - z[k - 1], h0 = z[k + 1] - z[k];
+            for (int f = 0; f < 4; f++) {
+                dz1[f] = (zp1[f] - zm1[f]) / (hm1 + h0);
+                dzz[f] = d2_mid(zm1[f], c[f], zp1[f], hm1, h0);
+            }
+        }
+        double uz = dz1[0], vz = dz1[1], wz = dz1[2], Tz = dz1[3];
+        double Txx = dxx[3], Tyy = dyy[3], Tzz = dzz[3];
+        double lap_u = dxx[0] + dyy[0] + dzz[0];
+        double lap_v = dxx[1] + dyy[1] + dzz[1];
+        double lap_w = dxx[2] + dyy[2] + dzz[2];
+        double lap_T = dxx[3] + dyy[3] + dzz[3];
+        /* strain of the modified tensor and its norm (src/cu/pde.cu:632-646) */
+        double S_11 = 2 * ux / 3 - (vy + wz) / 3;
+        double S_22 = 2 * vy / 3 - (ux + wz) / 3;
+        double S_33 = 2 * wz / 3 - (ux + vy) / 3;
+        double S_12 = 0.5 * (uy + vx);
+        double S_13 = 0.5 * (uz + wx);
+        double S_23 = 0.5 * (wy + vz);
+        double mod_S = sqrt(2.0 * (ux * ux + vy * vy + wz * wz) + (uz + wx) * (uz + wx) + (vx + uy) * (vx + uy) +
+                            (wy + vz) * (wy + vz));
+        double dz_k = (k < Nz - 1) ? z[k + 1] - z[k] : z[k] - z[k - 1];
+        /* Arrhenius rate and fuel (src/cu/pde.cu:664-688) */
+        double H_step = (T_ijk > T_pc) ? 1.0 : 0.0;
+        double K_T = A * exp(-T_act / T_ijk) * H_step;
+        double T_gas = T_inf, T_solid = T_inf;
+        int nzy = Nz_Y[IDX(i, j, 0, Nx, Ny, 1)];
+        if (k <= nzy) {
+            if (k < nzy) {
+                Y_ijk = R_old[Yi + IDX(i, j, k, Nx, Ny, Nzm)];
+                double Y_RHS = -Y_f * K_T * Y_ijk;
+                R_new[Yi + IDX(i, j, k, Nx, Ny, Nzm)] = Y_RHS;
+            } else {
+                T_gas = zp1[3];
+            }
+            if (Y_ijk > 0.0)
+                T_solid = T_ijk;
+            else
+                T_gas = T_ijk;
+        }
+        if (suth_mu == 1) mu = 1.716e-5 * pow(T_ijk / 273.15, 1.5) * (273.15 + 110.4) / (T_ijk + 110.4);
+        if (suth_k == 1) {
+            kT = 0.0241 * pow(T_ijk / 273.15, 1.5) * (273.15 + 194) / (T_ijk + 194);
+            dk = 0.0241 * (1.5 * pow(T_ijk / 273.15, 0.5) * (273.15 + 194) / (T_ijk + 194) -
+                           pow(T_ijk / 273.15, 1.5) * (273.15 + 194) / ((T_ijk + 194) * (T_ijk + 194)));
+        }
+        double nu = mu / rho_ijk;
+        double h_c = h_c0 * dz_k * pow(fabs(T_gas - T_solid) / dz_k, 1.0 / 4.0);
+        double q = H_C * Y_ijk * K_T / c_p - h_c * alpha_s * sigma_s * (T_gas - T_solid) / (c_p * T_inf * rho_inf / T_gas);
+        double T2 = T_ijk * T_ijk, T3 = T_ijk * T_ijk * T_ijk;
+        double T_RHS_tmp = (dk + 12 * O_SIGMA * delta * T2 * (Tx * Tx + Ty * Ty + Tz * Tz) +
+                            (kT + 4 * O_SIGMA * delta * T3) * lap_T) / (rho_ijk * c_p) + q;
+        double T_RHS;
+        if (conv == 0)
+            T_RHS = T_RHS_tmp - (u_ijk * Tx + v_ijk * Ty + w_ijk * Tz);
+        else
+            T_RHS = T_RHS_tmp - (uTx + vTy + wTz);
+        double mod_U = sqrt(u_ijk * u_ijk + v_ijk * v_ijk + w_ijk * w_ijk);
+        double F_x = -0.5 * C_d * alpha_s * sigma_s * Y_ijk * mod_U * u_ijk;
+        double F_y = -0.5 * C_d * alpha_s * sigma_s * Y_ijk * mod_U * v_ijk;
+        double F_z = g * (rho_ijk - rho_inf) / rho_ijk - 0.5 * C_d * alpha_s * sigma_s * Y_ijk * mod_U * w_ijk;
+        double u_RHS = nu * lap_u - (uux + vuy + wuz) + F_x;
+        double v_RHS = nu * lap_v - (uvx + vvy + wvz) + F_y;
+        double w_RHS = nu * lap_w - (uwx + vwy + wwz) + F_z;
+        double cdelta = C_s * pow(dx * dy * dz_k, 1.0 / 3.0);
+        double mu_sgs = rho_ijk * pow(cdelta, 2.0) * mod_S;
+        long q3 = IDX(i, j, k, Nx, Ny, Nz);
+        R_turb[ti.rho + q3] = rho_ijk;
+        R_turb[ti.Tx + q3] = Tx;
+        R_turb[ti.Ty + q3] = Ty;
+        R_turb[ti.Tz + q3] = Tz;
+        R_turb[ti.Txx + q3] = Txx;
+        R_turb[ti.Tyy + q3] = Tyy;
+        R_turb[ti.Tzz + q3] = Tzz;
+        R_turb[ti.mu_sgs + q3] = mu_sgs;
+        R_turb[ti.S_11 + q3] = S_11;
+        R_turb[ti.S_12 + q3] = S_12;
+        R_turb[ti.S_13 + q3] = S_13;
+        R_turb[ti.S_21 + q3] = S_12;
+        R_turb[ti.S_22 + q3] = S_22;
+        R_turb[ti.S_23 + q3] = S_23;
+        R_turb[ti.S_31 + q3] = S_13;
+        R_turb[ti.S_32 + q3] = S_23;
+        R_turb[ti.S_33 + q3] = S_33;
+        R_turb[ti.div_U + q3] = nu * T_RHS_tmp / T_ijk;
+        if (k == 0) {
+            double a1 = mu * (uz + wx * 0.0), a2 = mu * (vz + wy * 0.0);
+            double tau_w = sqrt(a1 * a1 + a2 * a2);
+            R_turb[ti.u_tau_wall + IDX(i, j, 0, Nx, Ny, 1)] = sqrt(tau_w / rho_ijk);
+        }
+        R_new[ui + q3] = u_RHS;
+        R_new[vi + q3] = v_RHS;
+        R_new[wi + q3] = w_RHS;
+        R_new[Ti + q3] = T_RHS;
+    }
+}
+
+/* ----------------------------------------------------------- turbulence ---- */
+
+/* Van Driest damping 1 - exp(-z_ibm u_tau / A_VD / nu) (src/cu/turbulence.cu:346). */
+static inline double fw_damp(double zibm, double utau, double nu) {
+    return 1 - exp(-zibm * utau / O_A_VD / nu);
+}
+
+/* src/cu/turbulence.cu:255-454.  [Q3] at k = 0 and k = Nz-1 the reference reads
+ * div_U values it never loads (turbulence.cu:402,417); the intended neighbours
+ * are used here.  Those nodes' u,v,w are overwritten by the boundary
+ * conditions, so the choice has no effect on any output. */
+void oracle_turbulence(const Parameters *P, const double *R_turb, double *R_new) {
+    const int Nx = P->Nx, Ny = P->Ny, Nz = P->Nz;
+    const long size = (long)Nx * Ny * Nz;
+    const double dx = P->dx, dy = P->dy, mu = P->mu, Pr = P->Pr;
+    const double *z = P->z, *z_ibm = P->z_ibm;
+    const TurbulenceIndexes ti = P->turbulence_indexes;
+    const FieldIndexes fi = P->field_indexes;
+
+#pragma omp parallel for schedule(static)
+    for (long ijk = 0; ijk < size; ijk++) {
+        int i = (int)(ijk / ((long)Ny * Nz));
+        int j = (int)((ijk % ((long)Ny * Nz)) / Nz);
+        int k = (int)(ijk % Nz);
+        int im1 = (i - 1 + Nx - 1) % (Nx - 1), jm1 = (j - 1 + Ny - 1) % (Ny - 1);
+        int ip1 = (i + 1) % (Nx - 1), jp1 = (j + 1) % (Ny - 1);
+        long c = IDX(i, j, k, Nx, Ny, Nz);
+        long cxm = IDX(im1, j, k, Nx, Ny, Nz), cxp = IDX(ip1, j, k, Nx, Ny, Nz);
+        long cym = IDX(i, jm1, k, Nx, Ny, Nz), cyp = IDX(i, jp1, k, Nx, Ny, Nz);
+        double rho = R_turb[ti.rho + c];
+        double Tx = R_turb[ti.Tx + c], Ty = R_turb[ti.Ty + c], Tz = R_turb[ti.Tz + c];
+        double Txx = R_turb[ti.Txx + c], Tyy = R_turb[ti.Tyy + c], Tzz = R_turb[ti.Tzz + c];
+        double mu_sgs_ijk = R_turb[ti.mu_sgs + c];
+        double S_11 = R_turb[ti.S_11 + c], S_12 = R_turb[ti.S_12 + c], S_13 = R_turb[ti.S_13 + c];
+        double S_21 = R_turb[ti.S_21 + c], S_22 = R_turb[ti.S_22 + c], S_23 = R_turb[ti.S_23 + c];
+        double S_31 = R_turb[ti.S_31 + c], S_32 = R_turb[ti.S_32 + c], S_33 = R_turb[ti.S_33 + c];
+        double mu_ip = R_turb[ti.mu_sgs + cxp], mu_im = R_turb[ti.mu_sgs + cxm];
+        double mu_jp = R_turb[ti.mu_sgs + cyp], mu_jm = R_turb[ti.mu_sgs + cym];
+        double S_11_ip = R_turb[ti.S_11 + cxp], S_11_im = R_turb[ti.S_11 + cxm];
+        double S_12_ip = R_turb[ti.S_12 + cxp], S_12_im = R_turb[ti.S_12 + cxm];
+        double S_13_ip = R_turb[ti.S_13 + cxp], S_13_im = R_turb[ti.S_13 + cxm];
+        double S_21_jp = R_turb[ti.S_21 + cyp], S_21_jm = R_turb[ti.S_21 + cym];
+        double S_22_jp = R_turb[ti.S_22 + cyp], S_22_jm = R_turb[ti.S_22 + cym];
+        double S_23_jp = R_turb[ti.S_23 + cyp], S_23_jm = R_turb[ti.S_23 + cym];
+        double dU_ip = R_turb[ti.div_U + cxp], dU_im = R_turb[ti.div_U + cxm];
+        double dU_jp = R_turb[ti.div_U + cyp], dU_jm = R_turb[ti.div_U + cym];
+        double ut = R_turb[ti.u_tau_wall + IDX(i, j, 0, Nx, Ny, 1)];
+        double ut_ip = R_turb[ti.u_tau_wall + IDX(ip1, j, 0, Nx, Ny, 1)];
+        double ut_im = R_turb[ti.u_tau_wall + IDX(im1, j, 0, Nx, Ny, 1)];
+        double ut_jp = R_turb[ti.u_tau_wall + IDX(i, jp1, 0, Nx, Ny, 1)];
+        double ut_jm = R_turb[ti.u_tau_wall + IDX(i, jm1, 0, Nx, Ny, 1)];
+        double mu_km1 = 0.0, S_31_km1 = 0.0, S_32_km1 = 0.0, S_33_km1 = 0.0, dU_km1 = 0.0;
+        double mu_kp1 = 0.0, S_31_kp1 = 0.0, S_32_kp1 = 0.0, S_33_kp1 = 0.0, dU_kp1 = 0.0;
+        double nu = mu / rho;
+        double fw = fw_damp(z_ibm[c], ut, nu);
+        double fw_ip = fw_damp(z_ibm[cxp], ut_ip, nu);
+        double fw_im = fw_damp(z_ibm[cxm], ut_im, nu);
+        double fw_jp = fw_damp(z_ibm[cyp], ut_jp, nu);
+        double fw_jm = fw_damp(z_ibm[cym], ut_jm, nu);
+        mu_sgs_ijk *= fw * fw;
+        mu_im *= fw_im * fw_im;
+        mu_ip *= fw_ip * fw_ip;
+        mu_jm *= fw_jm * fw_jm;
+        mu_jp *= fw_jp * fw_jp;
+        if (k > 0) {
+            long cz = IDX(i, j, k - 1, Nx, Ny, Nz);
+            double f = fw_damp(z_ibm[cz], ut, nu);
+            mu_km1 = R_turb[ti.mu_sgs + cz];
+            S_31_km1 = R_turb[ti.S_31 + cz];
+            S_32_km1 = R_turb[ti.S_32 + cz];
+            S_33_km1 = R_turb[ti.S_33 + cz];
+            dU_km1 = R_turb[ti.div_U + cz];
+            mu_km1 *= f * f;
+        }
+        if (k < Nz - 1) {
+            long cz = IDX(i, j, k + 1, Nx, Ny, Nz);
+            double f = fw_damp(z_ibm[cz], ut, nu);
+            mu_kp1 = R_turb[ti.mu_sgs + cz];
+            S_31_kp1 = R_turb[ti.S_31 + cz];
+            S_32_kp1 = R_turb[ti.S_32 + cz];
+            S_33_kp1 = R_turb[ti.S_33 + cz];
+            dU_kp1 = R_turb[ti.div_U + cz];
+            mu_kp1 *= f * f;
+        }
+        double mu_x = (mu_ip - mu_im) / (2 * dx);
+        double mu_y = (mu_jp - mu_jm) / (2 * dy);
+        double S_11_x = (S_11_ip - S_11_im) / (2.0 * dx);
+        double S_12_x = (S_12_ip - S_12_im) / (2.0 * dx);
+        double S_13_x = (S_13_ip - S_13_im) / (2.0 * dx);
+        double S_21_y = (S_21_jp - S_21_jm) / (2.0 * dy);
+        double S_22_y = (S_22_jp - S_22_jm) / (2.0 * dy);
+        double S_23_y = (S_23_jp - S_23_jm) / (2.0 * dy);
+        double dU_x = (dU_ip - dU_im) / (2.0 * dx);
+        double dU_y = (dU_jp - dU_jm) / (2.0 * dy);
+        double mu_z, S_31_z, S_32_z, S_33_z, dU_z;
+        if (k == 0) {
+            double h0 = z[k + 1] - z[k], h1 = z[k + 2] - z[k + 1];
+            long cz = IDX(i, j, k + 2, Nx, Ny, Nz);
+            double f = fw_damp(z_ibm[cz], ut, nu);
+            double mu_kp2 = R_turb[ti.mu_sgs + cz] * (f * f);
+            mu_z = d1_fwd(mu_sgs_ijk, mu_kp1, mu_kp2, h0, h1);
+            S_31_z = d1_fwd(S_31, S_31_kp1, R_turb[ti.S_31 + cz], h0, h1);
+            S_32_z = d1_fwd(S_32, S_32_kp1, R_turb[ti.S_32 + cz], h0, h1);
+            S_33_z = d1_fwd(S_33, S_33_kp1, R_turb[ti.S_33 + cz], h0, h1);
+            dU_z = d1_fwd(R_turb[ti.div_U + c], dU_kp1, R_turb[ti.div_U + cz], h0, h1);
+        } else if (k == Nz - 1) {
+            double hm1 = z[k] - z[k - 1], hm2 = z[k - 1] - z[k - 2];
+            long cz = IDX(i, j, k - 2, Nx, Ny, Nz);
+            double f = fw_damp(z_ibm[cz], ut, nu);
+            double mu_km2 = R_turb[ti.mu_sgs + cz] * (f * f);
+            mu_z = d1_bwd(mu_km2, mu_km1, mu_sgs_ijk, hm2, hm1);
+            S_31_z = d1_bwd(R_turb[ti.S_31 + cz], S_31_km1, S_31, hm2, hm1);
+            S_32_z = d1_bwd(R_turb[ti.S_32 + cz], S_32_km1, S_32, hm2, hm1);
+            S_33_z = d1_bwd(R_turb[ti.S_33 + cz], S_33_km1, S_33, hm2, hm1);
+            dU_z = d1_bwd(R_turb[ti.div_U + cz], dU_km1, R_turb[ti.div_U + c], hm2, hm1);
+        } else {
+            double hm1 = z[k] - z[k - 1], h0 = z[k + 1] - z[k];
+            mu_z = (mu_kp1 - mu_km1) / (h0 + hm1);
+            S_31_z = (S_31_kp1 - S_31_km1) / (h0 + hm1);
+            S_32_z = (S_32_kp1 - S_32_km1) / (h0 + hm1);
+            S_33_z = (S_33_kp1 - S_33_km1) / (h0 + hm1);
+            dU_z = (dU_kp1 - dU_km1) / (h0 + hm1);
+        }
+        double div_S_x = S_11_x + S_21_y + S_31_z;
+        double div_S_y = S_12_x + S_22_y + S_32_z;
+        double div_S_z = S_13_x + S_23_y + S_33_z;
+        double sgs_x = -2 * (mu_x * S_11 + mu_y * S_21 + mu_z * S_31 + mu_sgs_ijk * div_S_x) / rho;
+        double sgs_y = -2 * (mu_x * S_12 + mu_y * S_22 + mu_z * S_32 + mu_sgs_ijk * div_S_y) / rho;
+        double sgs_z = -2 * (mu_x * S_13 + mu_y * S_23 + mu_z * S_33 + mu_sgs_ijk * div_S_z) / rho;
+        double sgs_q = -(mu_x * Tx + mu_y * Ty + mu_z * Tz + mu_sgs_ijk * (Txx + Tyy + Tzz)) / (rho * Pr);
+        R_new[fi.u + c] += -sgs_x + dU_x / 3.0;
+        R_new[fi.v + c] += -sgs_y + dU_y / 3.0;
+        R_new[fi.w + c] += -sgs_z + dU_z / 3.0;
+        R_new[fi.T + c] -= sgs_q;
+    }
+}
+
+/* --------------------------------------------------- boundary conditions ---- */
+
+/* src/cu/pde.cu:14-99, one column at a time, in the reference's read/write order. */
+void oracle_boundary_conditions(const Parameters *P, double *R) {
+    const int Nx = P->Nx, Ny = P->Ny, Nz = P->Nz, Nzm = P->Nz_Y_max;
+    const double *z = P->z;
+    const FieldIndexes fi = P->field_indexes;
+#pragma omp parallel for schedule(static)
+    for (long ij = 0; ij < (long)Nx * Ny; ij++) {
+        int i = (int)(ij / Ny), j = (int)(ij % Ny);
+        int nzy = P->Nz_Y[IDX(i, j, 0, Nx, Ny, 1)];
+        int k = P->cut_nodes[IDX(i, j, 0, Nx, Ny, 1)];
+        double dz_k = z[k + 1] - z[k], dz_kp1 = z[k + 2] - z[k + 1];
+        double T1 = R[fi.T + IDX(i, j, k + 1, Nx, Ny, Nz)];
+        double T2 = R[fi.T + IDX(i, j, k + 2, Nx, Ny, Nz)];
+        double Y1 = (k + 1 < nzy) ? R[fi.Y + IDX(i, j, k + 1, Nx, Ny, Nzm)] : 0.0;
+        double Y2 = (k + 2 < nzy) ? R[fi.Y + IDX(i, j, k + 2, Nx, Ny, Nzm)] : 0.0;
+        R[fi.u + IDX(i, j, k, Nx, Ny, Nz)] = 0;
+        R[fi.v + IDX(i, j, k, Nx, Ny, Nz)] = 0;
+        R[fi.w + IDX(i, j, k, Nx, Ny, Nz)] = 0;
+        R[fi.T + IDX(i, j, k, Nx, Ny, Nz)] =
+            ((dz_k + dz_kp1) * (dz_k + dz_kp1) * T1 - dz_k * dz_k * T2) / (dz_kp1 * (2 * dz_k + dz_kp1));
+        if (k < nzy)
+            R[fi.Y + IDX(i, j, k, Nx, Ny, Nzm)] =
+                ((dz_k + dz_kp1) * (dz_k + dz_kp1) * Y1 - dz_k * dz_k * Y2) / (dz_kp1 * (2 * dz_k + dz_kp1));
+        k = Nz - 1;
+        double h1 = z[k] - z[k - 1], h2 = z[k - 1] - z[k - 2];
+        const int off[4] = {fi.u, fi.v, fi.w, fi.T};
+        double m1[4], m2[4];
+        for (int f = 0; f < 4; f++) {
+            m1[f] = R[off[f] + IDX(i, j, k - 1, Nx, Ny, Nz)];
+            m2[f] = R[off[f] + IDX(i, j, k - 2, Nx, Ny, Nz)];
+        }
+        for (int f = 0; f < 4; f++)
+            R[off[f] + IDX(i, j, k, Nx, Ny, Nz)] = ((h2 + h1) * (h2 + h1) * m1[f] - h1 * h1 * m2[f]) / (h2 * (h2 + 2 * h1));
+        int cut = P->cut_nodes[IDX(i, j, 0, Nx, Ny, 1)];
+        for (k = 0; k < cut; k++) {
+            R[fi.u + IDX(i, j, k, Nx, Ny, Nz)] = P->u_dead_nodes;
+            R[fi.v + IDX(i, j, k, Nx, Ny, Nz)] = P->v_dead_nodes;
+            R[fi.w + IDX(i, j, k, Nx, Ny, Nz)] = P->w_dead_nodes;
+            R[fi.T + IDX(i, j, k, Nx, Ny, Nz)] = P->T_dead_nodes;
+            if (k < nzy) R[fi.Y + IDX(i, j, k, Nx, Ny, Nzm)] = P->Y_dead_nodes;
+        }
+    }
+}
+
+/* src/cu/pde.cu:101-137 */
+void oracle_bounds(const Parameters *P, double *y) {
+    const int Nx = P->Nx, Ny = P->Ny, Nz = P->Nz, Nzm = P->Nz_Y_max;
+    const long size = (long)Nx * Ny * Nz;
+    const FieldIndexes fi = P->field_indexes;
+#pragma omp parallel for schedule(static)
+    for (long ijk = 0; ijk < size; ijk++) {
+        int i = (int)(ijk / ((long)Ny * Nz));
+        int j = (int)((ijk % ((long)Ny * Nz)) / Nz);
+        int k = (int)(ijk % Nz);
+        if (k < Nzm) {
+            double *Yp = &y[fi.Y + IDX(i, j, k, Nx, Ny, Nzm)];
+            if (*Yp < P->Y_min) *Yp = P->Y_min;
+            if (*Yp > P->Y_max) *Yp = P->Y_max;
+        }
+        double *Tp = &y[fi.T + IDX(i, j, k, Nx, Ny, Nz)];
+        if (*Tp < P->T_min) *Tp = P->T_min;
+        if (*Tp > P->T_max) *Tp = P->T_max;
+    }
+}
+
+/* ------------------------------------------------------------ projection ---- */
+
+/* src/cu/pde.cu:789-908 */
+void oracle_velocity_correction(const Parameters *P, double *y_np1, const double *y_n, const double *p) {
+    const int Nx = P->Nx, Ny = P->Ny, Nz = P->Nz, fd_z = P->velocity_correction_fd;
+    const long size = (long)Nx * Ny * Nz;
+    const double dx = P->dx, dy = P->dy, dt = P->dt, rho_inf = P->rho_inf, T_inf = P->T_inf;
+    const double *z = P->z;
+    const FieldIndexes fi = P->field_indexes;
+#pragma omp parallel for schedule(static)
+    for (long ijk = 0; ijk < size; ijk++) {
+        int i = (int)(ijk / ((long)Ny * Nz));
+        int j = (int)((ijk % ((long)Ny * Nz)) / Nz);
+        int k = (int)(ijk % Nz);
+        int im1 = (i - 1 + Nx - 1) % (Nx - 1), jm1 = (j - 1 + Ny - 1) % (Ny - 1);
+        int ip1 = (i + 1) % (Nx - 1), jp1 = (j + 1) % (Ny - 1);
+        long c = IDX(i, j, k, Nx, Ny, Nz);
+        double p_ijk = p[c];
+        double rho = T_inf * rho_inf / y_n[fi.T + c];
+        double px = (p[IDX(ip1, j, k, Nx, Ny, Nz)] - p[IDX(im1, j, k, Nx, Ny, Nz)]) / (2 * dx);
+        double py = (p[IDX(i, jp1, k, Nx, Ny, Nz)] - p[IDX(i, jm1, k, Nx, Ny, Nz)]) / (2 * dy);
+        double pz = 0.0;
+        int use_fwd, use_bwd;
+        if (fd_z == 0) {
+            use_fwd = (k == 0);
+            use_bwd = (k == Nz - 1);
+        } else if (fd_z == 1) {
+            use_fwd = (k < Nz - 2);
+            use_bwd = !use_fwd;
+        } else if (fd_z == -1) {
+            use_fwd = (k < 2);
+            use_bwd = !use_fwd;
+        } else {
+            use_fwd = use_bwd = 0;
+        }
+        if (use_fwd) {
+            double h0 = z[k + 1] - z[k], h1 = z[k + 2] - z[k + 1];
+            pz = d1_fwd(p_ijk, p[IDX(i, j, k + 1, Nx, Ny, Nz)], p[IDX(i, j, k + 2, Nx, Ny, Nz)], h0, h1);
+        } else if (use_bwd) {
+            double hm1 = z[k] - z[k - 1], hm2 = z[k - 1] - z[k - 2];
+            pz = d1_bwd(p[IDX(i, j, k - 2, Nx, Ny, Nz)], p[IDX(i, j, k - 1, Nx, Ny, Nz)], p_ijk, hm2, hm1);
+        } else if (fd_z == 0) {
+            double hm1 = z[k] - z[k - 1], h0 = z[k + 1] - z[k];
+            pz = (p[IDX(i, j, k + 1, Nx, Ny, Nz)] - p[IDX(i, j, k - 1, Nx, Ny, Nz)]) / (hm1 + h0);
+        }
+        if (P->variable_density == 0) rho = rho_inf;
+        y_np1[fi.u + c] -= dt * px / rho;
+        y_np1[fi.v + c] -= dt * py / rho;
+        y_np1[fi.w + c] -= dt * pz / rho;
+    }
+}
+
+/* ------------------------------------------------------------------- FFT ---- */
+
+static int smallest_factor(int n) {
+    if (n % 4 == 0) return 4;
+    if (n % 2 == 0) return 2;
+    for (int p = 3; p * p <= n; p += 2)
+        if (n % p == 0) return p;
+    return n;
+}
+
+/* Recursive mixed-radix decimation in time.  x: input at stride xs; X: output
+ * (contiguous, length n); tw: table of exp(sign*2*pi*i*e/Ntot) for e < Ntot;
+ * tstep = Ntot / n; tmp: scratch of at least n complex. */
+static void fft_rec(const double *x, long xs, double *X, int n, const double *tw, int Ntot, int tstep, double *tmp) {
+    if (n == 1) {
+        X[0] = x[0];
+        X[1] = x[1];
+        return;
+    }
+    int p = smallest_factor(n), m = n / p;
+    for (int r = 0; r < p; r++) fft_rec(x + 2 * r * xs, xs * p, X + 2 * r * m, m, tw, Ntot, tstep * p, tmp);
+    /* X[k + q m] = sum_r W_n^{r (k + q m)} Y_r[k] */
+    for (int kk = 0; kk < m; kk++) {
+        for (int q = 0; q < p; q++) {
+            double re = 0.0, im = 0.0;
+            long base = ((long)(kk + q * m) * tstep) % Ntot, e = 0;
+            for (int r = 0; r < p; r++, e += base) {
+                if (e >= Ntot) e -= Ntot;
+                double wr = tw[2 * e], wi = tw[2 * e + 1];
+                double yr = X[2 * (r * m + kk)], yi = X[2 * (r * m + kk) + 1];
+                re += wr * yr - wi * yi;
+                im += wr * yi + wi * yr;
+            }
+            tmp[2 * q] = re;
+            tmp[2 * q + 1] = im;
+        }
+        for (int q = 0; q < p; q++) {
+            X[2 * (kk + q * m)] = tmp[2 * q];
+            X[2 * (kk + q * m) + 1] = tmp[2 * q + 1];
+        }
+    }
+}
+
+static double *twiddle_table(int n, int sign) {
+    double *tw = (double *)malloc(sizeof(double) * 2 * (size_t)n);
+    for (int e = 0; e < n; e++) {
+        double ang = 2.0 * M_PI * (double)e / (double)n;
+        tw[2 * e] = cos(ang);
+        tw[2 * e + 1] = sign * sin(ang);
+    }
+    return tw;
+}
+
+/* Length-n transforms of `count` sequences (sequence q starts at data + q*dist
+ * complex elements, elements at `stride`), with a shared twiddle table. */
+static void fft_many(double *data, int n, long stride, long dist, long count, int sign) {
+    double *tw = twiddle_table(n, sign);
+#pragma omp parallel
+    {
+        double *in = (double *)malloc(sizeof(double) * 2 * (size_t)n);
+        double *out = (double *)malloc(sizeof(double) * 2 * (size_t)n);
+        double *tmp = (double *)malloc(sizeof(double) * 2 * (size_t)n);
+#pragma omp for schedule(static)
+        for (long q = 0; q < count; q++) {
+            double *d = data + 2 * q * dist;
+            for (int e = 0; e < n; e++) {
+                in[2 * e] = d[2 * e * stride];
+                in[2 * e + 1] = d[2 * e * stride + 1];
+            }
+            fft_rec(in, 1, out, n, tw, n, 1, tmp);
+            for (int e = 0; e < n; e++) {
+                d[2 * e * stride] = out[2 * e];
+                d[2 * e * stride + 1] = out[2 * e + 1];
+            }
+        }
+        free(in);
+        free(out);
+        free(tmp);
+    }
+    free(tw);
+}
+
+void oracle_fft(double *data, int n, int stride, int sign) { fft_many(data, n, stride, 0, 1, sign); }
+
+/* 2D DFT over (i, j) of `slabs` slabs in FFTWIDX layout (j fastest), as the
+ * batched plan of src/cu/pressure.cu:527 computes it. */
+static void fft2_slabs(double *data, int nx, int ny, long slabs, int sign) {
+    /* along j (contiguous rows) for every (i, k) */
+    fft_many(data, ny, 1, ny, (long)nx * slabs, sign);
+    /* along i (stride ny) for every (j, k) */
+    for (long k = 0; k < slabs; k++) fft_many(data + 2 * k * (long)nx * ny, nx, ny, 1, ny, sign);
+}
+
+/* ------------------------------------------------------------- pressure ---- */
+
+/* src/cu/pressure.cu:356-390 */
+void oracle_tridiagonal_coefficients(const Parameters *P, int r, int s, double *a, double *b, double *c) {
+    int Nz = P->Nz;
+    const double *z = P->z;
+    double K2 = P->kx[r] * P->kx[r] + P->ky[s] * P->ky[s];
+    for (int k = 0; k < Nz - 2; k++) {
+        double dz_k = z[k + 1] - z[k];
+        if (k == 0) {
+            double dz_kp1 = z[k + 2] - z[k + 1];
+            double dz_Nzm2 = z[Nz - 2] - z[Nz - 3];
+            double dz_Nzm1 = z[Nz - 1] - z[Nz - 2];
+            a[k] = 2 / (dz_k * (dz_k + dz_kp1));
+            b[k] = -1.0 / dz_k;
+            c[k] = (dz_k + dz_kp1) / (dz_k * dz_kp1) - 0.5 * dz_k * K2 - 1 / dz_kp1;
+            b[Nz - 2] = -(K2 + 2 / (dz_Nzm2 * dz_Nzm1));
+        } else {
+            double dz_km1 = z[k] - z[k - 1];
+            double dz_kp1 = z[k + 2] - z[k + 1];
+            a[k] = 2 / (dz_k * (dz_k + dz_kp1));
+            c[k] = 2 / (dz_k * (dz_km1 + dz_k));
+            b[k] = -(K2 + 2 / (dz_km1 * dz_k));
+        }
+    }
+}
+
+/* src/cu/pressure.cu:418-443.  [Q18] l and u are real because a, b, c are
+ * real; divisions by them are component-wise IEEE divisions. */
+void oracle_thomas(int n, const double *a, const double *b, const double *c, const double *d, double *x) {
+    double *u = (double *)malloc(sizeof(double) * (size_t)n);
+    double *l = (double *)malloc(sizeof(double) * (size_t)(n > 1 ? n - 1 : 1));
+    double *y = (double *)malloc(sizeof(double) * 2 * (size_t)n);
+    u[0] = b[0];
+    for (int k = 1; k < n; k++) {
+        l[k - 1] = a[k - 1] / u[k - 1];
+        u[k] = b[k] - l[k - 1] * c[k - 1];
+    }
+    y[0] = d[0];
+    y[1] = d[1];
+    for (int k = 1; k < n; k++) {
+        y[2 * k] = d[2 * k] - l[k - 1] * y[2 * (k - 1)];
+        y[2 * k + 1] = d[2 * k + 1] - l[k - 1] * y[2 * (k - 1) + 1];
+    }
+    x[2 * (n - 1)] = y[2 * (n - 1)] / u[n - 1];
+    x[2 * (n - 1) + 1] = y[2 * (n - 1) + 1] / u[n - 1];
+    for (int k = n - 2; k >= 0; k--) {
+        x[2 * k] = (y[2 * k] - c[k] * x[2 * (k + 1)]) / u[k];
+        x[2 * k + 1] = (y[2 * k + 1] - c[k] * x[2 * (k + 1) + 1]) / u[k];
+    }
+    free(u);
+    free(l);
+    free(y);
+}
+
+/* f of src/cu/pressure.cu:14-101 (constant density) or 103-327 (variable
+ * density) into f_hat (complex, slabs 1..Nz-2) and p_top_hat.  [Q17] slab 0 is
+ * not formed: update_coefficients reads slab 1 for its first equation
+ * (pressure.cu:492) and never reads slab 0 (whose Tzz in the reference reads an
+ * unset T_ijkp3, pressure.cu:231).  [Q1] h_c per cell with |T_gas - T_solid|. */
+static void pressure_rhs(oracle_ctx *ctx, const double *y_np1, const double *y_n, const double *p) {
+    const Parameters *P = &ctx->P;
+    const int Nx = P->Nx, Ny = P->Ny, Nz = P->Nz, Nzm = P->Nz_Y_max;
+    const double dx = P->dx, dy = P->dy, dt = P->dt, T_inf = P->T_inf, rho_inf = P->rho_inf;
+    const double delta = P->delta, c_p = P->c_p, alpha_s = P->alpha_s, sigma_s = P->sigma_s;
+    const double h_c0 = P->h_c, T_pc = P->T_pc, A = P->A, T_act = P->T_act, H_C = P->H_C;
+    const double *z = P->z;
+    const FieldIndexes fi = P->field_indexes;
+    double *f_hat = ctx->f_hat;
+    memset(f_hat, 0, sizeof(double) * 2 * (size_t)(Nx - 1) * (Ny - 1));
+#pragma omp parallel for schedule(static)
+    for (long ij = 0; ij < (long)(Nx - 1) * (Ny - 1); ij++) {
+        int i = (int)(ij / (Ny - 1)), j = (int)(ij % (Ny - 1));
+        long t = FIDX(i, j, 0, Nx - 1, Ny - 1);
+        ctx->p_top_hat[2 * t] = p[IDX(i, j, Nz - 1, Nx, Ny, Nz)];
+        ctx->p_top_hat[2 * t + 1] = 0.0;
+    }
+#pragma omp parallel for schedule(static)
+    for (long ijk = 0; ijk < (long)(Nx - 1) * (Ny - 1) * (Nz - 2); ijk++) {
+        int i = (int)(ijk / ((long)(Ny - 1) * (Nz - 2)));
+        int j = (int)((ijk % ((long)(Ny - 1) * (Nz - 2))) / (Nz - 2));
+        int k = 1 + (int)(ijk % (Nz - 2));
+        int im1 = (i - 1 + Nx - 1) % (Nx - 1), jm1 = (j - 1 + Ny - 1) % (Ny - 1);
+        int ip1 = (i + 1) % (Nx - 1), jp1 = (j + 1) % (Ny - 1);
+        long c = IDX(i, j, k, Nx, Ny, Nz);
+        double dz_km1 = z[k] - z[k - 1], dz_k = z[k + 1] - z[k];
+        double u_ijk = y_np1[fi.u + c], v_ijk = y_np1[fi.v + c];
+        double u_ip = y_np1[fi.u + IDX(ip1, j, k, Nx, Ny, Nz)], u_im = y_np1[fi.u + IDX(im1, j, k, Nx, Ny, Nz)];
+        double v_jp = y_np1[fi.v + IDX(i, jp1, k, Nx, Ny, Nz)], v_jm = y_np1[fi.v + IDX(i, jm1, k, Nx, Ny, Nz)];
+        double w_kp = y_np1[fi.w + IDX(i, j, k + 1, Nx, Ny, Nz)], w_km = y_np1[fi.w + IDX(i, j, k - 1, Nx, Ny, Nz)];
+        double wz = (w_kp - w_km) / (dz_k + dz_km1);
+        double u_iph = 0.5 * (u_ip + u_ijk), u_imh = 0.5 * (u_ijk + u_im);
+        double v_jph = 0.5 * (v_jp + v_ijk), v_jmh = 0.5 * (v_ijk + v_jm);
+        double ux = (u_iph - u_imh) / dx, vy = (v_jph - v_jmh) / dy;
+        double f;
+        if (P->variable_density == 0) {
+            f = rho_inf * (ux + vy + wz) / dt;
+        } else {
+            double kT = P->k, dk = 0.0;
+            double T_ijk = y_np1[fi.T + c];
+            double T_ip = y_np1[fi.T + IDX(ip1, j, k, Nx, Ny, Nz)], T_im = y_np1[fi.T + IDX(im1, j, k, Nx, Ny, Nz)];
+            double T_jp = y_np1[fi.T + IDX(i, jp1, k, Nx, Ny, Nz)], T_jm = y_np1[fi.T + IDX(i, jm1, k, Nx, Ny, Nz)];
+            double T_kp = y_np1[fi.T + IDX(i, j, k + 1, Nx, Ny, Nz)], T_km = y_np1[fi.T + IDX(i, j, k - 1, Nx, Ny, Nz)];
+            double rho_ijk = T_inf * rho_inf / y_n[fi.T + c];
+            double rho_ip = T_inf * rho_inf / y_n[fi.T + IDX(ip1, j, k, Nx, Ny, Nz)];
+            double rho_im = T_inf * rho_inf / y_n[fi.T + IDX(im1, j, k, Nx, Ny, Nz)];
+            double rho_jp = T_inf * rho_inf / y_n[fi.T + IDX(i, jp1, k, Nx, Ny, Nz)];
+            double rho_jm = T_inf * rho_inf / y_n[fi.T + IDX(i, jm1, k, Nx, Ny, Nz)];
+            double rho_kp = T_inf * rho_inf / y_n[fi.T + IDX(i, j, k + 1, Nx, Ny, Nz)];
+            double rho_km = T_inf * rho_inf / y_n[fi.T + IDX(i, j, k - 1, Nx, Ny, Nz)];
+            double Tx = (T_ip - T_im) / (2 * dx), Ty = (T_jp - T_jm) / (2 * dy);
+            double Txx = (T_ip - 2 * T_ijk + T_im) / (dx * dx), Tyy = (T_jp - 2 * T_ijk + T_jm) / (dy * dy);
+            double rhox = (rho_ip - rho_im) / (2 * dx), rhoy = (rho_jp - rho_jm) / (2 * dy);
+            double px = (p[IDX(ip1, j, k, Nx, Ny, Nz)] - p[IDX(im1, j, k, Nx, Ny, Nz)]) / (2 * dx);
+            double py = (p[IDX(i, jp1, k, Nx, Ny, Nz)] - p[IDX(i, jm1, k, Nx, Ny, Nz)]) / (2 * dy);
+            double rhoz = (rho_kp - rho_km) / (dz_k + dz_km1);
+            double pz = (p[IDX(i, j, k + 1, Nx, Ny, Nz)] - p[IDX(i, j, k - 1, Nx, Ny, Nz)]) / (dz_k + dz_km1);
+            double Tz = (T_kp - T_km) / (dz_k + dz_km1);
+            double Tzz = d2_mid(T_km, T_ijk, T_kp, dz_km1, dz_k);
+            double lap_T = Txx + Tyy + Tzz;
+            double Y_ijk = 0;
+            double H_step = (T_ijk > T_pc) ? 1.0 : 0.0;
+            double K_T = A * exp(-T_act / T_ijk) * H_step;
+            double T_gas = T_ijk, T_solid = T_ijk;
+            int nzy = P->Nz_Y[IDX(i, j, 0, Nx, Ny, 1)];
+            if (k <= nzy) {
+                if (k < nzy) Y_ijk = y_np1[fi.Y + IDX(i, j, k, Nx, Ny, Nzm)];
+                if (Y_ijk > 0.0)
+                    T_gas = T_pc + (T_ijk - T_pc);
+                else
+                    T_gas = T_ijk;
+            }
+            if (P->sutherland_conductivity == 1) {
+                kT = 0.0241 * pow(T_ijk / 273.15, 1.5) * (273.15 + 194) / (T_ijk + 194);
+                dk = 0.0241 * (1.5 * pow(T_ijk / 273.15, 0.5) * (273.15 + 194) / (T_ijk + 194) -
+                               pow(T_ijk / 273.15, 1.5) * (273.15 + 194) / ((T_ijk + 194) * (T_ijk + 194)));
+            }
+            double h_c = h_c0 * dz_k * pow(fabs(T_gas - T_solid) / dz_k, 1.0 / 4.0);
+            double q = H_C * Y_ijk * K_T / c_p - h_c * alpha_s * sigma_s * (T_gas - T_solid) / (c_p * T_inf * rho_inf / T_gas);
+            double T2 = T_ijk * T_ijk, T3 = T_ijk * T_ijk * T_ijk;
+            double div_U_temp = (dk + 12 * O_SIGMA * delta * T2 * (Tx * Tx + Ty * Ty + Tz * Tz) +
+                                 (kT + 4 * O_SIGMA * delta * T3) * lap_T) / (T_inf * rho_inf * c_p) + q / T_ijk;
+            f = rho_ijk * (ux + vy + wz - div_U_temp) / dt + (rhox * px + rhoy * py + rhoz * pz) / rho_ijk;
+        }
+        long t = FIDX(i, j, k, Nx - 1, Ny - 1);
+        f_hat[2 * t] = f;
+        f_hat[2 * t + 1] = 0.0;
+    }
+}
+
+/* One solve (src/cu/pressure.cu:506-549).  [Q6] duplicate planes i = Nx-1 and
+ * j = Ny-1 take the values of their images after the unique nodes are written. */
+static void pressure_solve_once(oracle_ctx *ctx, const double *y_np1, const double *y_n, double *p) {
+    const Parameters *P = &ctx->P;
+    const int Nx = P->Nx, Ny = P->Ny, Nz = P->Nz;
+    const int nx = Nx - 1, ny = Ny - 1, nz = Nz - 1;
+    const double *z = P->z;
+    pressure_rhs(ctx, y_np1, y_n, p);
+    fft2_slabs(ctx->p_top_hat, nx, ny, 1, -1);
+    fft2_slabs(ctx->f_hat, nx, ny, nz, -1);
+    double dz0 = z[1] - z[0];
+    double dz_Nzm2 = z[Nz - 2] - z[Nz - 3], dz_Nzm1 = z[Nz - 1] - z[Nz - 2];
+    double top_div = 2 / (dz_Nzm1 * (dz_Nzm2 + dz_Nzm1));
+#pragma omp parallel
+    {
+        double *a = (double *)malloc(sizeof(double) * (size_t)nz);
+        double *b = (double *)malloc(sizeof(double) * (size_t)nz);
+        double *c = (double *)malloc(sizeof(double) * (size_t)nz);
+        double *d = (double *)malloc(sizeof(double) * 2 * (size_t)nz);
+        double *x = (double *)malloc(sizeof(double) * 2 * (size_t)nz);
+#pragma omp for schedule(static)
+        for (long rs = 0; rs < (long)nx * ny; rs++) {
+            int r = (int)(rs / ny), s = (int)(rs % ny);
+            oracle_tridiagonal_coefficients(P, r, s, a, b, c);
+            /* update_coefficients (src/cu/pressure.cu:473-504) */
+            long t1 = FIDX(r, s, 1, nx, ny), tl = FIDX(r, s, Nz - 2, nx, ny), tt = FIDX(r, s, 0, nx, ny);
+            d[0] = 0.5 * dz0 * ctx->f_hat[2 * t1];
+            d[1] = 0.5 * dz0 * ctx->f_hat[2 * t1 + 1];
+            for (int k = 1; k < Nz - 2; k++) {
+                long t = FIDX(r, s, k, nx, ny);
+                d[2 * k] = ctx->f_hat[2 * t];
+                d[2 * k + 1] = ctx->f_hat[2 * t + 1];
+            }
+            d[2 * (Nz - 2)] = ctx->f_hat[2 * tl] - ctx->p_top_hat[2 * tt] / top_div;
+            d[2 * (Nz - 2) + 1] = ctx->f_hat[2 * tl + 1] - ctx->p_top_hat[2 * tt + 1] / top_div;
+            oracle_thomas(nz, a, b, c, d, x);
+            for (int k = 0; k < nz; k++) {
+                long t = FIDX(r, s, k, nx, ny);
+                ctx->f_hat[2 * t] = x[2 * k];
+                ctx->f_hat[2 * t + 1] = x[2 * k + 1];
+            }
+        }
+        free(a);
+        free(b);
+        free(c);
+        free(d);
+        free(x);
+    }
+    fft2_slabs(ctx->f_hat, nx, ny, nz, +1);
+    double scale = (double)nx * ny;
+#pragma omp parallel for schedule(static)
+    for (long ijk = 0; ijk < (long)nx * ny * nz; ijk++) {
+        int i = (int)(ijk / ((long)ny * nz));
+        int j = (int)((ijk % ((long)ny * nz)) / nz);
+        int k = (int)(ijk % nz);
+        p[IDX(i, j, k, Nx, Ny, Nz)] = ctx->f_hat[2 * FIDX(i, j, k, nx, ny)] / scale;
+    }
+#pragma omp parallel for schedule(static)
+    for (long jk = 0; jk < (long)Ny * Nz; jk++) {
+        int j = (int)(jk / Nz), k = (int)(jk % Nz);
+        p[IDX(Nx - 1, j, k, Nx, Ny, Nz)] = p[IDX(0, j, k, Nx, Ny, Nz)];
+    }
+#pragma omp parallel for schedule(static)
+    for (long ik = 0; ik < (long)Nx * Nz; ik++) {
+        int i = (int)(ik / Nz), k = (int)(ik % Nz);
+        p[IDX(i, Ny - 1, k, Nx, Ny, Nz)] = p[IDX(i, 0, k, Nx, Ny, Nz)];
+    }
+}
+
+/* src/cu/pressure.cu:551-606; [Q5] the convergence measure is the true
+ * infinity norm of p - p_old over all Nx*Ny*Nz nodes. */
+void oracle_pressure(oracle_ctx *ctx, const double *y_np1, const double *y_n, double *p, double *error,
+                     int *iterations, int *solves) {
+    const Parameters *P = &ctx->P;
+    long N = ctx->N;
+    if (P->variable_density == 0) {
+        pressure_solve_once(ctx, y_np1, y_n, p);
+        ctx->solves++;
+        if (error) *error = 0.0;
+        if (iterations) *iterations = 0;
+        if (solves) *solves = 1;
+        return;
+    }
+    double tol = 0.0;
+    int m, nsolve = 0;
+    for (m = 0; m < P->pressure_solver_iter; m++) {
+        memcpy(ctx->p_tmp, p, sizeof(double) * (size_t)N);
+        pressure_solve_once(ctx, y_np1, y_n, p);
+        nsolve++;
+        double mx = 0.0;
+#pragma omp parallel for reduction(max : mx) schedule(static)
+        for (long q = 0; q < N; q++) {
+            double d = fabs(p[q] - ctx->p_tmp[q]);
+            if (d > mx) mx = d;
+        }
+        tol = mx;
+        if (tol <= P->pressure_solver_tol) break;
+    }
+    ctx->solves += nsolve;
+    if (error) *error = tol;
+    if (iterations) *iterations = m;
+    if (solves) *solves = nsolve;
+}
+
+/* ------------------------------------------------------------ time step ---- */
+
+/* src/cu/functions.cu:325-403 */
+static void temperature_source_step(oracle_ctx *ctx, double *y, double t_n) {
+    const Parameters *P = &ctx->P;
+    const int Nx = P->Nx, Ny = P->Ny, Nz = P->Nz;
+    const long N = ctx->N;
+    const int Ti = P->field_indexes.T;
+    if (P->t_source_start <= 0) {
+        if (!ctx->have_T_source) return;
+#pragma omp parallel for schedule(static)
+        for (long q = 0; q < N; q++)
+            if (ctx->T_source[q] > y[Ti + q]) y[Ti + q] = ctx->T_source[q];
+        return;
+    }
+    double scale = 1.0;
+    if (t_n <= P->t_source_start) scale = t_n / P->t_source_start;
+    if (t_n > P->t_source_end) return;
+    int gauss = strcmp(P->T_source_shape, "gaussian") == 0;
+    int box = strcmp(P->T_source_shape, "parallelepiped") == 0;
+#pragma omp parallel for schedule(static)
+    for (long ijk = 0; ijk < N; ijk++) {
+        int i = (int)(ijk / ((long)Ny * Nz));
+        int j = (int)((ijk % ((long)Ny * Nz)) / Nz);
+        int k = (int)(ijk % Nz);
+        double shape = 0.0;
+        if (gauss)
+            shape = exp(-pow((P->x[i] - P->TS_x_center) / P->TS_length, 2.0) -
+                        pow((P->y[j] - P->TS_y_center) / P->TS_width, 2.0) -
+                        pow((P->z[k] - P->TS_z_center) / P->TS_height, 2.0));
+        else if (box)
+            shape = (P->x[i] >= P->TS_x_start && P->x[i] <= P->TS_x_end && P->y[j] >= P->TS_y_start &&
+                     P->y[j] <= P->TS_y_end && P->z[k] >= P->TS_z_start && P->z[k] <= P->TS_z_end)
+                        ? 1.0
+                        : 0.0;
+        double temperature = scale * (P->T_inf + (P->T_source - P->T_inf) * shape);
+        if (temperature > y[Ti + IDX(i, j, k, Nx, Ny, Nz)]) y[Ti + IDX(i, j, k, Nx, Ny, Nz)] = temperature;
+    }
+}
+
+oracle_ctx *oracle_create(const Parameters *P) {
+    oracle_ctx *ctx = (oracle_ctx *)calloc(1, sizeof(oracle_ctx));
+    ctx->P = *P;
+    ctx->N = (long)P->Nx * P->Ny * P->Nz;
+    ctx->S = 4 * ctx->N + (long)P->Nx * P->Ny * P->Nz_Y_max;
+    ctx->M = (long)(P->Nx - 1) * (P->Ny - 1) * (P->Nz - 1);
+    /* [Q15] stage derivatives start at zero, so the Y entries the reference's
+     * RHS never writes (k >= Nz_Y of the column) stay zero. */
+    ctx->k = (double *)calloc(4 * (size_t)ctx->S, sizeof(double));
+    ctx->F = (double *)calloc((size_t)ctx->S, sizeof(double));
+    ctx->y_np1 = (double *)calloc((size_t)ctx->S, sizeof(double));
+    ctx->R_turb = (double *)calloc(18 * (size_t)ctx->N + (size_t)P->Nx * P->Ny, sizeof(double));
+    ctx->f_hat = (double *)calloc(2 * (size_t)ctx->M, sizeof(double));
+    ctx->p_top_hat = (double *)calloc(2 * (size_t)(P->Nx - 1) * (P->Ny - 1), sizeof(double));
+    ctx->p_tmp = (double *)calloc((size_t)ctx->N, sizeof(double));
+    ctx->T_source = (double *)calloc((size_t)ctx->N, sizeof(double));
+    return ctx;
+}
+
+void oracle_destroy(oracle_ctx *ctx) {
+    if (!ctx) return;
+    free(ctx->k);
+    free(ctx->F);
+    free(ctx->y_np1);
+    free(ctx->R_turb);
+    free(ctx->f_hat);
+    free(ctx->p_top_hat);
+    free(ctx->p_tmp);
+    free(ctx->T_source);
+    free(ctx);
+}
+
+long oracle_pressure_solves(const oracle_ctx *ctx) { return ctx->solves; }
+
+void oracle_phi(oracle_ctx *ctx, const double *R_old, double *R_new) {
+    oracle_rhs(&ctx->P, R_old, R_new, ctx->R_turb);
+    oracle_turbulence(&ctx->P, ctx->R_turb, R_new);
+    oracle_boundary_conditions(&ctx->P, R_new);
+}
+
+/* c = y + a x (src/cu/utils.cu:14-21) */
+static void axpy(double *c, const double *x, const double *y, double a, long n) {
+#pragma omp parallel for schedule(static)
+    for (long q = 0; q < n; q++) c[q] = y[q] + a * x[q];
+}
+
+/* src/cu/solver.cu:14-93 and 178-265 */
+int oracle_step(oracle_ctx *ctx, double *y_n, double *p, int n) {
+    const Parameters *P = &ctx->P;
+    const long S = ctx->S;
+    const double dt = P->dt;
+    double *k1 = ctx->k, *k2 = ctx->k + S, *k3 = ctx->k + 2 * S, *k4 = ctx->k + 3 * S;
+    double *F = ctx->F, *y1 = ctx->y_np1;
+    if (!ctx->have_T_source) {
+        memcpy(ctx->T_source, y_n + P->field_indexes.T, sizeof(double) * (size_t)ctx->N);
+        ctx->have_T_source = 1;
+    }
+    if (strncmp(P->method, "Euler", 5) == 0) {
+        oracle_phi(ctx, y_n, F);
+#pragma omp parallel for schedule(static)
+        for (long q = 0; q < S; q++) y1[q] = y_n[q] + dt * F[q];
+    } else if (strncmp(P->method, "RK2", 3) == 0) {
+        oracle_phi(ctx, y_n, k1);
+        axpy(F, k1, y_n, dt, S);
+        oracle_phi(ctx, F, k2);
+#pragma omp parallel for schedule(static)
+        for (long q = 0; q < S; q++) y1[q] = y_n[q] + 0.5 * dt * (k1[q] + k2[q]);
+    } else if (strncmp(P->method, "RK4", 3) == 0) {
+        oracle_phi(ctx, y_n, k1);
+        axpy(F, k1, y_n, dt * 0.5, S);
+        oracle_phi(ctx, F, k2);
+        axpy(F, k2, y_n, dt * 0.5, S);
+        oracle_phi(ctx, F, k3);
+        axpy(F, k3, y_n, dt, S);
+        oracle_phi(ctx, F, k4);
+#pragma omp parallel for schedule(static)
+        for (long q = 0; q < S; q++) y1[q] = y_n[q] + (dt / 6) * (k1[q] + 2 * k2[q] + 2 * k3[q] + k4[q]);
+    } else {
+        return -1;
+    }
+    oracle_pressure(ctx, y1, y_n, p, NULL, NULL, NULL);
+    oracle_velocity_correction(P, y1, y_n, p);
+    oracle_boundary_conditions(P, y1);
+    oracle_bounds(P, y1);
+    if (P->t[n] <= P->t_source_end) temperature_source_step(ctx, y1, P->t[n]);
+    memcpy(y_n, y1, sizeof(double) * (size_t)S);
+    return 0;
+}
+
+int oracle_run(oracle_ctx *ctx, double *y_n, double *p, int step0, int n_steps) {
+    for (int n = step0; n < step0 + n_steps; n++) {
+        int rc = oracle_step(ctx, y_n, p, n);
+        if (rc != 0) return rc;
+    }
+    return 0;
+}

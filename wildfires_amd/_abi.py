@@ -1,0 +1,143 @@
+"""ctypes view of the C-ABI declared in include/wildfires.h.
+
+The Parameters structure is built from the header itself (member names, types
+and order), so the Python side cannot drift from the C side; load() checks the
+size against wf_parameters_sizeof().
+"""
+import ctypes
+import os
+import re
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "wildfires.h")
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libwildfires_amd.so")
+
+_CTYPES = {
+    "int": ctypes.c_int,
+    "double": ctypes.c_double,
+    "size_t": ctypes.c_size_t,
+    "FILE": ctypes.c_void_p,
+}
+
+
+def _struct_body(text, tag):
+    m = re.search(r"typedef struct %s \{(.*?)\} (\w+);" % re.escape(tag), text, re.S)
+    if not m:
+        raise RuntimeError("struct %s not found in %s" % (tag, HEADER))
+    return m.group(1), m.group(2)
+
+
+def _make_struct(text, tag, known):
+    body, name = _struct_body(text, tag)
+    fields = []
+    for line in body.splitlines():
+        line = line.split("/*")[0].strip()
+        if not line or not line.endswith(";"):
+            continue
+        decl = line[:-1].strip()
+        m = re.match(r"(\w+)\s*(\*?)\s*(\w+)(\[(\d+)\])?$", decl)
+        if not m:
+            raise RuntimeError("cannot parse member '%s'" % decl)
+        ctype_name, ptr, member, _, count = m.groups()
+        if ctype_name in known:
+            base = known[ctype_name]
+        else:
+            base = _CTYPES[ctype_name]
+        if ptr:
+            base = ctypes.c_void_p if ctype_name == "FILE" else ctypes.POINTER(base)
+        if ctype_name == "char" and count:
+            base = ctypes.c_char * int(count)
+        elif count:
+            base = base * int(count)
+        fields.append((member, base))
+    cls = type(name, (ctypes.Structure,), {"_fields_": fields})
+    known[name] = cls
+    return cls
+
+
+def _build_types():
+    with open(HEADER) as f:
+        text = f.read()
+    known = {"char": ctypes.c_char}
+    for tag in ("_field_indexes", "_pressure_indexes", "_turbulence_indexes", "_log_files"):
+        _make_struct(text, tag, known)
+    params = _make_struct(text, "_parameters", known)
+    return params, known
+
+
+Parameters, STRUCTS = _build_types()
+PARAMS_P = ctypes.POINTER(Parameters)
+DBL_P = ctypes.POINTER(ctypes.c_double)
+INT_P = ctypes.POINTER(ctypes.c_int)
+
+# (name, restype, argtypes) of every exported entry point of include/wildfires.h
+EXPORTS = [
+    ("read_parameters_file", Parameters, [ctypes.c_char_p]),
+    ("free_parameters", None, [Parameters]),
+    ("initial_conditions", None, [DBL_P] * 6 + [Parameters]),
+    ("initial_conditions_input", None, [DBL_P] * 6 + [Parameters]),
+    ("create_y_0", None, [DBL_P] * 6 + [Parameters]),
+    ("save_data", None, [DBL_P, DBL_P, ctypes.c_int, Parameters]),
+    ("save_domain", None, [Parameters]),
+    ("save_topography", None, [Parameters]),
+    ("log_parameters_report", None, [Parameters, ctypes.c_int]),
+    ("log_parameters", None, [Parameters]),
+    ("log_message", None, [Parameters, ctypes.c_char_p]),
+    ("log_timestep", None, [Parameters, ctypes.c_int] + [ctypes.c_double] * 7),
+    ("timestep_reports", None, [DBL_P] * 6 + [ctypes.c_int, Parameters]),
+    ("solve_PDE", None, [DBL_P, DBL_P, Parameters]),
+    ("fft_freq", None, [DBL_P, ctypes.c_int, ctypes.c_double]),
+    ("equispaced_domain", None, [DBL_P, ctypes.c_int, ctypes.c_double, ctypes.c_double]),
+    ("non_equispaced_domain", None, [DBL_P, ctypes.c_int] + [ctypes.c_double] * 3),
+    ("transition_domain", None, [DBL_P, ctypes.c_int] + [ctypes.c_double] * 3 + [ctypes.c_int]),
+    ("flat_terrain", None, [PARAMS_P]),
+    ("simple_hill", None, [PARAMS_P]),
+    ("ibm_parameters", None, [PARAMS_P]),
+    ("power_law", ctypes.c_double, [ctypes.c_double] * 4),
+    ("gaussian", ctypes.c_double, [ctypes.c_double] * 9),
+    ("parallelepiped", ctypes.c_double, [ctypes.c_double] * 9),
+    ("wf_last_error", ctypes.c_char_p, []),
+    ("wf_version", ctypes.c_char_p, []),
+    ("wf_parameters_read", PARAMS_P, [ctypes.c_char_p]),
+    ("wf_parameters_free", None, [PARAMS_P]),
+    ("wf_parameters_sizeof", ctypes.c_size_t, []),
+    ("wf_state_size", ctypes.c_long, [PARAMS_P]),
+    ("wf_solver_create", ctypes.c_int, [PARAMS_P, ctypes.POINTER(ctypes.c_void_p)]),
+    ("wf_solver_destroy", ctypes.c_int, [ctypes.c_void_p]),
+    ("wf_solver_set_state", ctypes.c_int, [ctypes.c_void_p, DBL_P, DBL_P]),
+    ("wf_solver_get_state", ctypes.c_int, [ctypes.c_void_p, DBL_P, DBL_P]),
+    ("wf_solver_step", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]),
+    ("wf_solver_phi", ctypes.c_int, [ctypes.c_void_p, DBL_P, DBL_P]),
+    ("wf_solver_pressure", ctypes.c_int, [ctypes.c_void_p, DBL_P, DBL_P, DBL_P, DBL_P, INT_P]),
+    ("wf_solver_project", ctypes.c_int, [ctypes.c_void_p, DBL_P, DBL_P, DBL_P, ctypes.c_int]),
+    ("wf_solver_stats", ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_long),
+                                       ctypes.POINTER(ctypes.c_long), DBL_P]),
+    ("wf_solver_profile", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    ("wf_solver_kernel_times", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, DBL_P, ctypes.POINTER(ctypes.c_long)]),
+    ("wf_solver_last_pressure", ctypes.c_int, [ctypes.c_void_p, DBL_P, INT_P]),
+    ("wf_device_synchronize", ctypes.c_int, []),
+]
+
+_lib = None
+
+
+def load(path=None):
+    """Load libwildfires_amd.so and declare every entry point.  Raises
+    ImportError when the built library is missing: there is no fallback."""
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    path = path or LIB_PATH
+    if not os.path.exists(path):
+        raise ImportError("libwildfires_amd.so is not built (%s); run __graft_entry__.build()" % path)
+    lib = ctypes.CDLL(path)
+    for name, res, args in EXPORTS:
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    size = lib.wf_parameters_sizeof()
+    if size != ctypes.sizeof(Parameters):
+        raise ImportError("Parameters layout mismatch: C %d bytes, Python %d bytes" % (size, ctypes.sizeof(Parameters)))
+    if path == LIB_PATH:
+        _lib = lib
+    return lib

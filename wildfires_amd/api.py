@@ -1,0 +1,255 @@
+"""Python mirror of the C-ABI (include/wildfires.h) over ctypes.
+
+The reference's interface for this path is C (src/cu/main.cu:27-76); these
+wrappers keep its names and argument meaning and add numpy views of the
+Parameters arrays.  Every call goes to libwildfires_amd.so; a missing library
+raises ImportError and a failing wf_* call raises WildfireError.
+"""
+import contextlib
+import ctypes
+import os
+
+import numpy as np
+
+from . import _abi
+
+
+class WildfireError(RuntimeError):
+    def __init__(self, code, message):
+        super().__init__("wf error %d: %s" % (code, message))
+        self.code = code
+
+
+def lib():
+    return _abi.load()
+
+
+def _check(rc):
+    if rc != 0:
+        raise WildfireError(rc, lib().wf_last_error().decode(errors="replace"))
+
+
+def _dp(a):
+    if a.dtype != np.float64 or not a.flags["C_CONTIGUOUS"]:
+        raise ValueError("expected a C-contiguous float64 array")
+    return a.ctypes.data_as(_abi.DBL_P)
+
+
+@contextlib.contextmanager
+def _output_root(root):
+    old = os.environ.get("WILDFIRES_OUTPUT_ROOT")
+    if root is not None:
+        os.environ["WILDFIRES_OUTPUT_ROOT"] = str(root)
+    try:
+        yield
+    finally:
+        if root is not None:
+            if old is None:
+                os.environ.pop("WILDFIRES_OUTPUT_ROOT", None)
+            else:
+                os.environ["WILDFIRES_OUTPUT_ROOT"] = old
+
+
+class Params:
+    """A Parameters structure read by read_parameters_file (owned, freed on close)."""
+
+    def __init__(self, path, output_root=None):
+        with _output_root(output_root):
+            self.ptr = lib().wf_parameters_read(str(path).encode())
+        if not self.ptr:
+            raise WildfireError(-1, "cannot read %s" % path)
+        self.p = self.ptr.contents
+
+    def close(self):
+        if getattr(self, "ptr", None):
+            lib().wf_parameters_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __getattr__(self, name):
+        if name in ("ptr", "p"):
+            raise AttributeError(name)
+        return getattr(self.p, name)
+
+    def _arr(self, name, count, dtype=np.float64):
+        ptr = getattr(self.p, name)
+        return np.ctypeslib.as_array(ptr, shape=(count,)).view(dtype)
+
+    @property
+    def N(self):
+        return self.p.Nx * self.p.Ny * self.p.Nz
+
+    @property
+    def S(self):
+        return int(lib().wf_state_size(self.ptr))
+
+    @property
+    def x(self):
+        return self._arr("x", self.p.Nx)
+
+    @property
+    def y(self):
+        return self._arr("y", self.p.Ny)
+
+    @property
+    def z(self):
+        return self._arr("z", self.p.Nz)
+
+    @property
+    def t(self):
+        return self._arr("t", self.p.Nt + 1)
+
+    @property
+    def kx(self):
+        return self._arr("kx", self.p.Nx - 1)
+
+    @property
+    def ky(self):
+        return self._arr("ky", self.p.Ny - 1)
+
+    @property
+    def topography(self):
+        return self._arr("topography", self.p.Nx * self.p.Ny)
+
+    @property
+    def z_ibm(self):
+        return self._arr("z_ibm", self.N)
+
+    @property
+    def cut_nodes(self):
+        return self._arr("cut_nodes", self.p.Nx * self.p.Ny, np.int32)
+
+    @property
+    def Nz_Y(self):
+        return self._arr("Nz_Y", self.p.Nx * self.p.Ny, np.int32)
+
+    @property
+    def save_path(self):
+        return self.p.save_path.decode()
+
+    def split(self, y):
+        """Views (u, v, w, T, Y) of a state vector, shaped (Nx, Ny, Nz) and (Nx, Ny, Nz_Y_max)."""
+        Nx, Ny, Nz, Nzm = self.p.Nx, self.p.Ny, self.p.Nz, self.p.Nz_Y_max
+        N = Nx * Ny * Nz
+        f = [y[q * N:(q + 1) * N].reshape(Nx, Ny, Nz) for q in range(4)]
+        f.append(y[4 * N:4 * N + Nx * Ny * Nzm].reshape(Nx, Ny, Nzm))
+        return f
+
+
+def read_parameters(path, output_root=None):
+    return Params(path, output_root)
+
+
+def initial_conditions(params):
+    """Analytic initial state of src/cu/functions.cu:187-323: (y_0, p)."""
+    P = params.p
+    N = params.N
+    NY = P.Nx * P.Ny * P.Nz_Y_max
+    u, v, w, T, p = (np.zeros(N) for _ in range(5))
+    Y = np.zeros(max(NY, 1))
+    lib().initial_conditions(_dp(u), _dp(v), _dp(w), _dp(T), _dp(Y), _dp(p), P)
+    y0 = np.zeros(params.S)
+    lib().create_y_0(_dp(u), _dp(v), _dp(w), _dp(T), _dp(Y), _dp(y0), P)
+    return y0, p
+
+
+def pack_state(params, u, v, w, T, Y):
+    """State vector [u|v|w|T|Y] from fields in the reference layout."""
+    y0 = np.zeros(params.S)
+    N = params.N
+    y0[:N] = np.ravel(u)
+    y0[N:2 * N] = np.ravel(v)
+    y0[2 * N:3 * N] = np.ravel(w)
+    y0[3 * N:4 * N] = np.ravel(T)
+    y0[4 * N:] = np.ravel(Y)
+    return y0
+
+
+class Solver:
+    """Device solver for one parameter set (wf_solver_*)."""
+
+    def __init__(self, params):
+        self.params = params
+        self.handle = ctypes.c_void_p()
+        _check(lib().wf_solver_create(params.ptr, ctypes.byref(self.handle)))
+
+    def close(self):
+        if self.handle:
+            lib().wf_solver_destroy(self.handle)
+            self.handle = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def set_state(self, y, p):
+        y = np.ascontiguousarray(y, dtype=np.float64)
+        p = np.ascontiguousarray(p, dtype=np.float64)
+        _check(lib().wf_solver_set_state(self.handle, _dp(y), _dp(p)))
+
+    def get_state(self):
+        y = np.zeros(self.params.S)
+        p = np.zeros(self.params.N)
+        _check(lib().wf_solver_get_state(self.handle, _dp(y), _dp(p)))
+        return y, p
+
+    def step(self, step0, n_steps):
+        _check(lib().wf_solver_step(self.handle, int(step0), int(n_steps)))
+
+    def phi(self, R_old):
+        R_old = np.ascontiguousarray(R_old, dtype=np.float64)
+        R_new = np.zeros_like(R_old)
+        _check(lib().wf_solver_phi(self.handle, _dp(R_old), _dp(R_new)))
+        return R_new
+
+    def pressure(self, y_np1, y_n, p):
+        p = np.array(p, dtype=np.float64, copy=True)
+        err = ctypes.c_double(0.0)
+        it = ctypes.c_int(0)
+        _check(lib().wf_solver_pressure(self.handle, _dp(np.ascontiguousarray(y_np1)), _dp(np.ascontiguousarray(y_n)),
+                                        _dp(p), ctypes.byref(err), ctypes.byref(it)))
+        return p, err.value, it.value
+
+    def project(self, y_np1, y_n, p, n):
+        y = np.array(y_np1, dtype=np.float64, copy=True)
+        _check(lib().wf_solver_project(self.handle, _dp(y), _dp(np.ascontiguousarray(y_n)),
+                                       _dp(np.ascontiguousarray(p)), int(n)))
+        return y
+
+    def stats(self):
+        it = ctypes.c_long(0)
+        st = ctypes.c_long(0)
+        err = ctypes.c_double(0.0)
+        _check(lib().wf_solver_stats(self.handle, ctypes.byref(it), ctypes.byref(st), ctypes.byref(err)))
+        return {"pressure_solves": it.value, "steps": st.value, "last_error": err.value}
+
+    def profile(self, enable=True):
+        _check(lib().wf_solver_profile(self.handle, 1 if enable else 0))
+
+    def kernel_times(self):
+        names = ["derived", "stage", "pressure_rhs_fx", "pressure_fy", "thomas", "pressure_inverse_fx",
+                 "project", "other"]
+        out = {}
+        for kind, name in enumerate(names):
+            ms = ctypes.c_double(0.0)
+            n = ctypes.c_long(0)
+            _check(lib().wf_solver_kernel_times(self.handle, kind, ctypes.byref(ms), ctypes.byref(n)))
+            out[name] = (ms.value, n.value)
+        return out
+
+
+def synchronize():
+    _check(lib().wf_device_synchronize())

@@ -1,0 +1,64 @@
+"""Build of libwildfires_amd.so (host C front end + gfx950 HIP kernels) and of
+the `wildfires_amd` command-line driver, in-tree.
+
+    python -m wildfires_amd.build        # or __graft_entry__.build()
+"""
+import os
+import subprocess
+import sys
+import concurrent.futures as cf
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(PKG)
+CSRC = os.path.join(PKG, "csrc")
+OUT = os.path.join(PKG, "build")
+LIB = os.path.join(PKG, "libwildfires_amd.so")
+EXE = os.path.join(PKG, "wildfires_amd")
+
+HOST_SOURCES = ["params.c", "terrain.c", "init.c", "output.c", "solver.c"]
+HIP_SOURCES = ["wf_stage.hip", "wf_pressure.hip", "wf_project.hip", "wf_solver.hip"]
+
+# FP64 kernels are compiled without FMA contraction (expressions evaluate in
+# the written order) and without packed FP32 instructions.
+HIPCC_FLAGS = ["--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17", "-ffp-contract=off",
+               "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+               "-Wno-unused-result", "-I" + os.path.join(ROOT, "include")]
+GCC_FLAGS = ["-O2", "-fPIC", "-std=gnu11", "-ffp-contract=off", "-fno-fast-math", "-Wall",
+             "-Wno-unused-result", "-Wno-format-truncation", "-I" + os.path.join(ROOT, "include")]
+
+
+def _hipcc():
+    return os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+
+
+def _run(cmd, verbose):
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+
+
+def build(verbose=False, jobs=None):
+    os.makedirs(OUT, exist_ok=True)
+    jobs = jobs or min(8, os.cpu_count() or 4)
+    cmds, objs = [], []
+    for src in HOST_SOURCES:
+        obj = os.path.join(OUT, src + ".o")
+        cmds.append(["gcc"] + GCC_FLAGS + ["-c", os.path.join(CSRC, "host", src), "-o", obj])
+        objs.append(obj)
+    for src in HIP_SOURCES:
+        obj = os.path.join(OUT, src + ".o")
+        cmds.append([_hipcc()] + HIPCC_FLAGS + ["-c", os.path.join(CSRC, "hip", src), "-o", obj])
+        objs.append(obj)
+    with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
+        list(ex.map(lambda c: _run(c, verbose), cmds))
+    _run([_hipcc(), "--offload-arch=gfx950", "-shared", "-fPIC", "-o", LIB + ".tmp"] + objs + ["-lm"], verbose)
+    os.replace(LIB + ".tmp", LIB)
+    main_obj = os.path.join(OUT, "main.c.o")
+    _run(["gcc"] + GCC_FLAGS + ["-c", os.path.join(CSRC, "host", "main.c"), "-o", main_obj], verbose)
+    _run(["gcc", "-o", EXE + ".tmp", main_obj, "-L" + PKG, "-lwildfires_amd", "-Wl,-rpath,$ORIGIN", "-lm"], verbose)
+    os.replace(EXE + ".tmp", EXE)
+    return LIB
+
+
+if __name__ == "__main__":
+    print(build(verbose="-v" in sys.argv))

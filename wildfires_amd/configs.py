@@ -1,0 +1,152 @@
+"""Parameter files and initial states of the benchmark and test workloads.
+
+Every workload is expressed with the reference's own parameter keys
+(src/cu/parameters.cu:79-414), so the same files drive the reference's driver
+and this library's `wildfires_amd` driver.
+
+  flat64   : 64x64x32 on 200x200x20 m, flat terrain, power-law wind u_r = 5 m/s,
+             T_inf = 300 K, a 4x4x0.5 m ignition block at (50,100) m, RK4,
+             dt = 0.01 s, 500 steps, parser defaults otherwise (Y_f = 0).
+  hill256  : 256x256x128 on 400x400x50 m, Gaussian hill of height 20 m and
+             standard deviation 50 m at (200,200) m, fuel layer 0.5 m above the
+             terrain with Y ~ U[0.8, 1.0] (seeded), power-law wind u_r = 6 m/s,
+             a line ignition T = 1000 K for 99 <= x <= 101 m over all y up to
+             1 m above the terrain, RK4, dt = 0.005 s, 1000 steps.
+Burning workloads use A = 5e5 1/s, Y_f = 10 and T_max = 1500 K (FIRE_KEYS).
+With the parser defaults (A = 1e9) the Arrhenius heat release of a 1000 K
+fuel node grows by ~5e3 1/s, 27 times the RK4 stability limit at dt = 0.005 s:
+the stage temperatures reach 1e16 K and the state is NaN after one step.
+These values are the largest tested that stay finite for 1000 steps.
+  small_*  : the same physics on grids the CPU oracle steps in well under a
+             second, used by the parity tests.
+"""
+import math
+import os
+
+import numpy as np
+
+BASE_KEYS = {
+    "t_min": 0.0,
+    "x_min": 0.0,
+    "y_min": 0.0,
+    "z_min": 0.0,
+    "method": "RK4",
+    "z_domain": 0,
+    "U0_type": "power_law",
+    "z_r": 2.0,
+    "alpha_u": 1.0 / 7.0,
+    "T0_shape": "parallelepiped",
+    "T_0": 1000.0,
+    "p_top": 0.0,
+    "time_step_log": 100,
+}
+
+# hill "length"/"width" are the divisors of (x - x_c) in exp(-((x-x_c)/L)^2 ...)
+# (src/cu/functions.cu:22-24), so a standard deviation of 50 m needs L = 50*sqrt(2).
+HILL_L = 50.0 * math.sqrt(2.0)
+
+FIRE_KEYS = {"A": 5e5, "Y_f": 10.0, "T_max": 1500.0}
+
+WORKLOADS = {
+    "flat64": dict(Nx=64, Ny=64, Nz=32, x_max=200.0, y_max=200.0, z_max=20.0, t_max=5.0, Nt=500, NT=500,
+                   topo_shape="flat", T_inf=300.0, u_r=5.0, T0_x_start=48.0, T0_x_end=52.0, T0_y_start=98.0,
+                   T0_y_end=102.0, T0_z_start=0.0, T0_z_end=0.5, Y_h=0.5, Y0_x_start=0.0, Y0_x_end=200.0,
+                   Y0_y_start=0.0, Y0_y_end=200.0, Y0_xa=1.0, Y0_ya=1.0),
+    "hill256": dict(Nx=256, Ny=256, Nz=128, x_max=400.0, y_max=400.0, z_max=50.0, t_max=5.0, Nt=1000, NT=1000,
+                    topo_shape="hill", hill_center_x=200.0, hill_center_y=200.0, hill_length=HILL_L,
+                    hill_width=HILL_L, hill_height=20.0, u_r=6.0, Y_h=0.5, Y0_x_start=0.0,
+                    Y0_x_end=400.0, Y0_y_start=0.0, Y0_y_end=400.0, Y0_xa=1.0, Y0_ya=1.0),
+    # parity cases: burning, with and without terrain
+    "small_hill": dict(Nx=24, Ny=20, Nz=16, x_max=48.0, y_max=40.0, z_max=8.0, t_max=0.2, Nt=20, NT=20,
+                       topo_shape="hill", hill_center_x=24.0, hill_center_y=20.0, hill_length=8.0,
+                       hill_width=7.0, hill_height=2.5, u_r=3.0, Y_h=1.0, Y0_x_start=0.0,
+                       Y0_x_end=48.0, Y0_y_start=0.0, Y0_y_end=40.0, Y0_xa=1.0, Y0_ya=1.0, T0_x_start=8.0,
+                       T0_x_end=12.0, T0_y_start=8.0, T0_y_end=30.0, T0_z_start=0.0, T0_z_end=1.5),
+    "small_flat": dict(Nx=20, Ny=24, Nz=14, x_max=40.0, y_max=48.0, z_max=7.0, t_max=0.2, Nt=20, NT=20,
+                       topo_shape="flat", u_r=4.0, Y_h=1.2, Y0_x_start=0.0, Y0_x_end=40.0,
+                       Y0_y_start=0.0, Y0_y_end=48.0, Y0_xa=1.0, Y0_ya=1.0, T0_x_start=10.0, T0_x_end=14.0,
+                       T0_y_start=16.0, T0_y_end=30.0, T0_z_start=0.0, T0_z_end=2.0),
+}
+
+
+BURNING = ("hill256", "small_hill", "small_flat")
+
+
+def parameter_text(name, **overrides):
+    keys = dict(BASE_KEYS)
+    if name in BURNING:
+        keys.update(FIRE_KEYS)
+    keys.update(WORKLOADS[name])
+    keys.update(overrides)
+    lines = []
+    for k, v in keys.items():
+        if isinstance(v, float):
+            v = repr(v)
+        lines.append("%s = %s" % (k, v))
+    return "\n".join(lines) + "\n"
+
+
+def write_parameter_file(path, name, **overrides):
+    with open(path, "w") as f:
+        f.write(parameter_text(name, **overrides))
+    return path
+
+
+def hill256_initial_state(params, seed=20261202, ignition_x=(99.0, 101.0), ignition_depth=1.0,
+                          Y_range=(0.8, 1.0), u_r=None):
+    """Initial state of the hill workload in the reference layout: (y_0, p).
+
+    Power-law wind u = u_r (z/z_r)^(1/7) over absolute height, v = w = 0,
+    T = T_inf except T = 1000 K for ignition_x[0] <= x <= ignition_x[1] and
+    z <= topography + ignition_depth, Y ~ U[Y_range] (numpy default_rng(seed))
+    on every stored fuel node, dead-node values below the terrain, p = 0."""
+    P = params.p
+    Nx, Ny, Nz, Nzm = P.Nx, P.Ny, P.Nz, P.Nz_Y_max
+    x, y, z = params.x, params.y, params.z
+    topo = params.topography.reshape(Nx, Ny)
+    cut = params.cut_nodes.reshape(Nx, Ny)
+    nzy = params.Nz_Y.reshape(Nx, Ny)
+    u_r = P.u_r if u_r is None else u_r
+    u = np.broadcast_to(u_r * (z / P.z_r) ** P.alpha_u, (Nx, Ny, Nz)).copy()
+    v = np.zeros((Nx, Ny, Nz))
+    w = np.zeros((Nx, Ny, Nz))
+    T = np.full((Nx, Ny, Nz), P.T_inf)
+    xs = (x >= ignition_x[0]) & (x <= ignition_x[1])
+    zz = z[None, None, :] <= (topo[:, :, None] + ignition_depth)
+    T[xs[:, None, None] & zz] = P.T_0
+    rng = np.random.default_rng(seed)
+    Y = rng.uniform(Y_range[0], Y_range[1], size=(Nx, Ny, Nzm))
+    kk = np.arange(Nzm)[None, None, :]
+    Y[kk >= nzy[:, :, None]] = 0.0
+    kz = np.arange(Nz)[None, None, :]
+    dead = kz < cut[:, :, None]
+    u[dead] = P.u_dead_nodes
+    v[dead] = P.v_dead_nodes
+    w[dead] = P.w_dead_nodes
+    T[dead] = P.T_dead_nodes
+    deady = (kk < cut[:, :, None]) & (kk < nzy[:, :, None])
+    Y[deady] = P.Y_dead_nodes
+    N = Nx * Ny * Nz
+    y0 = np.zeros(4 * N + Nx * Ny * Nzm)
+    y0[:N] = u.ravel()
+    y0[N:2 * N] = v.ravel()
+    y0[2 * N:3 * N] = w.ravel()
+    y0[3 * N:4 * N] = T.ravel()
+    y0[4 * N:] = Y.ravel()
+    p = np.zeros(N)
+    return y0, p
+
+
+def write_input_files(directory, params, y0, p):
+    """Write u,v,w,T,Y,p .bin files for the reference's input_path loader
+    (src/cu/input.cu:3-58)."""
+    os.makedirs(directory, exist_ok=True)
+    u, v, w, T, Y = (np.ascontiguousarray(a) for a in _split(params, y0))
+    for name, arr in (("u", u), ("v", v), ("w", w), ("T", T), ("Y", Y), ("p", p)):
+        np.asarray(arr, dtype="<f8").tofile(os.path.join(directory, name + ".bin"))
+
+
+def _split(params, y0):
+    P = params.p
+    N = P.Nx * P.Ny * P.Nz
+    return y0[:N], y0[N:2 * N], y0[2 * N:3 * N], y0[3 * N:4 * N], y0[4 * N:]

@@ -1,0 +1,115 @@
+/*
+ * wf_common.h -- device-side definitions shared by the HIP kernels.
+ *
+ * Internal layout (device): every scalar field is stored x-fastest,
+ * q = i + Nx*(j + Ny*k), so a wavefront's 64 lanes read 64 consecutive nodes
+ * of one x-row.  The fuel fraction Y keeps only its Nz_Y_max lowest planes.
+ * The reference layout (z fastest) exists only at the C-ABI boundary and is
+ * converted by the layout kernels.
+ *
+ * All FP64 kernels are compiled without FMA contraction so that every
+ * expression is evaluated in the operation order written, which is the order
+ * of the reference's expressions (src/cu/pde.cu, turbulence.cu, pressure.cu).
+ */
+#ifndef WF_COMMON_H
+#define WF_COMMON_H
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define WF_SIGMA_D 5.670374419e-8
+#define WF_A_VD_D 26.0
+
+/* Scalar constants and grid tables every kernel needs; passed by value. */
+struct WfConst {
+    int Nx, Ny, Nz, Nzm;      /* Nzm = Nz_Y_max */
+    long plane;               /* Nx*Ny */
+    long N;                   /* Nx*Ny*Nz */
+    double dx, dy, dt;
+    double mu, kT, Y_f, H_C, A, T_act, T_pc, h_c, alpha_s, sigma_s;
+    double T_inf, c_p, rho_inf, C_d, g, delta, Pr;
+    double T_min, T_max, Y_min, Y_max;
+    double u_dead, v_dead, w_dead, T_dead, Y_dead;
+    int temperature_convection, suth_mu, suth_k, variable_density, fd_z;
+    int exchange_on; /* alpha_s * sigma_s != 0: the h_c exchange term is evaluated */
+    const double *z;       /* Nz */
+    const double *cd2;     /* Nz: (C_s (dx dy dz_k)^(1/3))^2 evaluated on the host */
+    const double *topo;    /* plane */
+    const int *cut;        /* plane */
+    const int *nzy;        /* plane */
+};
+
+/* Device view of a state vector [u|v|w|T|Y] in the internal layout. */
+struct WfState {
+    double *u, *v, *w, *T, *Y;
+};
+
+/* Fields the RHS pass hands to its SGS divergence: the reference's
+ * R_turbulence planes that are read at a neighbour (src/cu/turbulence.cu:318-342). */
+struct WfDerived {
+    double *mu, *s11, *s12, *s13, *s22, *s23, *s33, *divU;
+    double *utau; /* plane */
+};
+
+__device__ __forceinline__ long wf_q(const WfConst &c, int i, int j, int k) {
+    return (long)i + (long)c.Nx * ((long)j + (long)c.Ny * (long)k);
+}
+
+__device__ __forceinline__ int wf_wrap_m(int i, int d, int n) { return (i - d + n - 1) % (n - 1); }
+__device__ __forceinline__ int wf_wrap_p(int i, int d, int n) { return (i + d) % (n - 1); }
+
+/* Second-order one-sided first derivative from nodes k, k+1, k+2
+ * (h0 = z[k+1]-z[k], h1 = z[k+2]-z[k+1]); src/cu/pde.cu:363-365. */
+__device__ __forceinline__ double wf_d1f(double f0, double f1, double f2, double h0, double h1) {
+    return -(2 * h0 + h1) * f0 / (h0 * (h0 + h1)) + (h0 + h1) * f1 / (h0 * h1) - h0 * f2 / (h1 * (h0 + h1));
+}
+
+/* Second-order one-sided first derivative from nodes k-2, k-1, k
+ * (hm2 = z[k-1]-z[k-2], hm1 = z[k]-z[k-1]); src/cu/pde.cu:383-385. */
+__device__ __forceinline__ double wf_d1b(double fm2, double fm1, double f0, double hm2, double hm1) {
+    return hm1 * fm2 / (hm2 * (hm2 + hm1)) - (hm2 + hm1) * fm1 / (hm2 * hm1) + (hm2 + 2 * hm1) * f0 / (hm1 * (hm2 + hm1));
+}
+
+/* Four-point second derivatives at the bottom / top nodes; src/cu/pde.cu:497-500, 539-542. */
+__device__ __forceinline__ double wf_d2f(double f0, double f1, double f2, double f3, double h0, double h1, double h2) {
+    return (6 * h0 + 4 * h1 + 2 * h2) * f0 / (h0 * (h0 + h1) * (h0 + h1 + h2)) -
+           (4 * h0 + 4 * h1 + 2 * h2) * f1 / (h0 * h1 * (h1 + h2)) +
+           (4 * h0 + 2 * h1 + 2 * h2) * f2 / (h1 * h2 * (h0 + h1)) -
+           (4 * h0 + 2 * h1) * f3 / (h2 * (h1 + h2) * (h0 + h1 + h2));
+}
+
+__device__ __forceinline__ double wf_d2b(double fm3, double fm2, double fm1, double f0, double hm3, double hm2, double hm1) {
+    return -(2 * hm2 + 4 * hm1) * fm3 / (hm3 * (hm3 + hm2) * (hm3 + hm2 + hm1)) +
+           (2 * hm3 + 2 * hm2 + 4 * hm1) * fm2 / (hm3 * hm2 * (hm2 + hm1)) -
+           (2 * hm3 + 4 * hm2 + 4 * hm1) * fm1 / (hm2 * hm1 * (hm3 + hm2)) +
+           (2 * hm3 + 4 * hm2 + 6 * hm1) * f0 / (hm1 * (hm2 + hm1) * (hm3 + hm2 + hm1));
+}
+
+/* Interior second derivative on a non-uniform grid; src/cu/pde.cu:572. */
+__device__ __forceinline__ double wf_d2m(double fm1, double f0, double fp1, double hm1, double h0) {
+    return 2 * fm1 / (hm1 * (hm1 + h0)) - 2 * f0 / (hm1 * h0) + 2 * fp1 / (h0 * (hm1 + h0));
+}
+
+/* Arrhenius rate A exp(-T_act/T) H(T - T_pc).  The reference multiplies by the
+ * step function; for T <= T_pc that product is +0, which is returned without
+ * evaluating exp. */
+__device__ __forceinline__ double wf_rate(const WfConst &c, double T) {
+    return (T > c.T_pc) ? c.A * exp(-c.T_act / T) * 1.0 : 0.0;
+}
+
+/* Van Driest damping 1 - exp(-z_ibm u_tau / A_VD / nu).  For an exponent below
+ * -38, exp() < 2^-54 and 1 - exp() rounds to 1 exactly, so exp is skipped. */
+__device__ __forceinline__ double wf_fw(double zibm, double utau, double nu) {
+    double a = -zibm * utau / WF_A_VD_D / nu;
+    return (a < -38.0) ? 1.0 : 1 - exp(a);
+}
+
+/* Distance to the terrain max(z_k - topo, 0) (src/cu/ibm.cu:84). */
+__device__ __forceinline__ double wf_zibm(const WfConst &c, int col, int k) {
+    double d = c.z[k] - c.topo[col];
+    return (d > 0.0) ? d : 0.0;
+}
+
+#define WF_CHECK_LAUNCH() (hipGetLastError())
+
+#endif

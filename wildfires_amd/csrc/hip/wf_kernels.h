@@ -1,0 +1,119 @@
+/*
+ * wf_kernels.h -- kernel launchers and device helpers shared between the
+ * stage, pressure and projection translation units.
+ */
+#ifndef WF_KERNELS_H
+#define WF_KERNELS_H
+
+#include "wf_common.h"
+
+/* k_stage modes (RK combination written by the stage pass) */
+#define WF_MODE_PHI 0   /* out = k_s */
+#define WF_MODE_FIRST 1 /* acc = k_1, out = y_n + a k_1 */
+#define WF_MODE_MID 2   /* acc += 2 k_s, out = y_n + a k_s */
+#define WF_MODE_LAST 3  /* out = y_n + a (acc + k_s) */
+#define WF_MODE_EULER 4 /* out = y_n + a k_s */
+
+/* Thermal conductivity by Sutherland's law and its T-derivative
+ * (src/cu/pde.cu:692-695). */
+__device__ __forceinline__ void wf_sutherland_k(double T, double *kT, double *dk) {
+    *kT = 0.0241 * pow(T / 273.15, 1.5) * (273.15 + 194) / (T + 194);
+    *dk = 0.0241 * (1.5 * pow(T / 273.15, 0.5) * (273.15 + 194) / (T + 194) -
+                    pow(T / 273.15, 1.5) * (273.15 + 194) / ((T + 194) * (T + 194)));
+}
+
+/* Heat source q of the temperature RHS (src/cu/pde.cu:663-699): combustion
+ * H_C Y K(T)/c_p minus the gas-solid exchange, with the gas/solid temperatures
+ * of the fuel zone k <= Nz_Y.  The exchange term is h_c alpha_s sigma_s (...),
+ * identically +-0 when alpha_s sigma_s = 0, in which case it is not formed
+ * (q - (+-0) = q).  h_c is evaluated per node from the parameter value.
+ * *Yout receives the node's fuel fraction (0 outside the stored fuel). */
+__device__ __forceinline__ double wf_heat_source_y(const WfConst &c, int col, int i, int j, int k, double T,
+                                                   double T_kp1, const double *Yf, double *Yout) {
+    double K_T = wf_rate(c, T);
+    double T_gas = c.T_inf, T_solid = c.T_inf, Y = 0.0;
+    int nzy = c.nzy[col];
+    if (k <= nzy) {
+        if (k < nzy)
+            Y = Yf[(long)col + c.plane * (long)k];
+        else
+            T_gas = T_kp1;
+        if (Y > 0.0)
+            T_solid = T;
+        else
+            T_gas = T;
+    }
+    double q = c.H_C * Y * K_T / c.c_p;
+    if (c.exchange_on) {
+        const double *z = c.z;
+        double dz_k = (k < c.Nz - 1) ? z[k + 1] - z[k] : z[k] - z[k - 1];
+        double h_c = c.h_c * dz_k * pow(fabs(T_gas - T_solid) / dz_k, 1.0 / 4.0);
+        q = q - h_c * c.alpha_s * c.sigma_s * (T_gas - T_solid) / (c.c_p * c.T_inf * c.rho_inf / T_gas);
+    }
+    *Yout = Y;
+    return q;
+}
+
+__device__ __forceinline__ double wf_heat_source(const WfConst &c, int col, int i, int j, int k, double T,
+                                                 double T_kp1, const double *Yf) {
+    double Y;
+    return wf_heat_source_y(c, col, i, j, k, T, T_kp1, Yf, &Y);
+}
+
+/* stage pass (wf_stage.hip) */
+hipError_t wf_launch_derived(const WfConst &c, const WfState &X, const WfDerived &D, hipStream_t s);
+hipError_t wf_launch_stage(const WfConst &c, const WfState &X, const WfDerived &D, const WfState &Yn,
+                           const WfState &acc, const WfState &out, int mode, double a, hipStream_t s);
+hipError_t wf_launch_to_internal(const double *src, double *dst, int Nx, int Ny, int nz, hipStream_t s);
+hipError_t wf_launch_to_reference(const double *src, double *dst, int Nx, int Ny, int nz, hipStream_t s);
+
+/* pressure pass (wf_pressure.hip) */
+struct WfSpectral {
+    int nx, ny, nz; /* unique nodes per direction: Nx-1, Ny-1, Nz-1 */
+    int hx;         /* nx/2 + 1 retained x-wavenumbers */
+    int fx_pairs;   /* row pairs per workgroup in the x-transforms */
+    int fy_cols;    /* kx columns per workgroup in the y-transform */
+    double2 *F;     /* [nz][ny][hx] spectral working buffer */
+    double2 *Ptop;  /* [ny][hx] spectrum of the top pressure plane */
+    double *A;      /* Nx*Ny*Nz: p-independent part of the Poisson RHS */
+    const double *U;     /* [nz][ny][hx]: Thomas pivots u_k */
+    const double *kx2;   /* hx: kx^2 of the retained x-wavenumbers */
+    const double *ky2;   /* ny */
+    const double *ta;    /* nz-1: sub-diagonal a_k (k-only) */
+    const double *tc;    /* nz-1: super-diagonal c_k for k >= 1 (c_0 depends on K^2) */
+    const double2 *twx;  /* nx twiddles exp(-2 pi i e / nx) */
+    const double2 *twy;  /* ny twiddles exp(-2 pi i e / ny) */
+    const int *radx;     /* radices of nx, 0-terminated (<= 16 entries) */
+    const int *rady;
+    double dz0, top_div, c0a, c0b; /* c_0 = c0a - 0.5 dz0 K^2 - c0b */
+    double b0;                     /* b_0 = -1/dz0 */
+    double beta_top;               /* 2/(dz_{Nz-2} dz_{Nz-1}) */
+    const double *beta;            /* nz: 2/(dz_{k-1} dz_k) for 1 <= k <= nz-2 */
+    unsigned long long *norm_bits; /* max |p_new - p_old| as ordered bits */
+};
+
+hipError_t wf_launch_pressure_rhs_fx(const WfConst &c, const WfSpectral &sp, const WfState &Ys, const WfState &Yn,
+                                     const double *p, int first_iter, hipStream_t s);
+hipError_t wf_launch_ptop_fx(const WfConst &c, const WfSpectral &sp, const double *p, hipStream_t s);
+hipError_t wf_launch_fy(const WfSpectral &sp, double2 *buf, int slab0, int nslabs, int sign, hipStream_t s);
+hipError_t wf_launch_thomas(const WfSpectral &sp, hipStream_t s);
+hipError_t wf_launch_pivots(const WfSpectral &sp, double *U, hipStream_t s);
+hipError_t wf_launch_inverse_fx(const WfConst &c, const WfSpectral &sp, double *p, hipStream_t s);
+hipError_t wf_launch_top_duplicates(const WfConst &c, const WfSpectral &sp, double *p, hipStream_t s);
+
+/* Temperature source applied at the end of a step (src/cu/functions.cu:325-403). */
+struct WfSource {
+    int mode;  /* 0 none, 1 max(T, T at t_0), 2 delayed shape */
+    int shape; /* mode 2: 1 gaussian, 2 parallelepiped, 0 none */
+    double scale, T_inf, T_source;
+    double xc, yc, zc, sx, sy, sz;
+    double x0, x1, y0, y1, z0, z1;
+    const double *x, *y; /* node coordinates */
+    const double *T_src; /* mode 1: T at the start of the run (internal layout) */
+};
+
+/* projection pass (wf_project.hip) */
+hipError_t wf_launch_project(const WfConst &c, const WfState &Ys, const WfState &Yn, const double *p,
+                             const WfState &out, const WfSource &src, hipStream_t s);
+
+#endif

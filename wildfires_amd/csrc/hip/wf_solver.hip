@@ -1,0 +1,671 @@
+/*
+ * wf_solver.hip -- host side of the time step: device workspace, the step
+ * sequence of src/cu/solver.cu:178-265 and the wf_* entry points.
+ *
+ * Device buffers (internal layout, see wf_common.h):
+ *   state[4]  four state vectors of S doubles: y_n, two stage inputs, and the
+ *             running RK combination; roles rotate between steps
+ *   derived   8 N + Nx Ny: per-node fields the SGS divergence reads at neighbours
+ *   p, A      pressure and the p-independent part of the Poisson RHS
+ *   F, Ptop   x-half-spectrum working buffer [Nz-1][Ny-1][(Nx-1)/2+1] and the
+ *             spectrum of the top pressure plane; U the Thomas pivots
+ */
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <vector>
+
+#include "../../../include/wildfires.h"
+#include "wf_common.h"
+#include "wf_fft.h"
+#include "wf_kernels.h"
+
+#define WF_VERSION "wildfires_amd 0.1 (gfx950, FP64)"
+#define WF_MAX_EVENTS 512
+
+static thread_local char wf_err[512] = "";
+
+static int wf_fail(int code, const char *fmt, ...) {
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(wf_err, sizeof(wf_err), fmt, ap);
+    va_end(ap);
+    return code;
+}
+
+#define WF_HIP(call)                                                                              \
+    do {                                                                                          \
+        hipError_t e_ = (call);                                                                   \
+        if (e_ != hipSuccess)                                                                     \
+            return wf_fail(WF_E_DEVICE, "%s:%d %s: %s", __FILE__, __LINE__, #call, hipGetErrorString(e_)); \
+    } while (0)
+
+extern "C" const char *wf_last_error(void) { return wf_err; }
+extern "C" const char *wf_version(void) { return WF_VERSION; }
+
+struct wf_solver {
+    Parameters P;
+    WfConst c;
+    WfSpectral sp;
+    long N, S, plane;
+    hipStream_t stream;
+    double *state[4];
+    int i_yn, i_x0, i_x1, i_acc;
+    double *derived;
+    double *p, *A, *U;
+    double2 *F, *Ptop;
+    double *z, *cd2, *topo, *kx2, *ky2, *ta, *tc, *beta, *x, *y;
+    int *cut, *nzy, *radx, *rady;
+    double2 *twx, *twy;
+    double *scratch, *Tsrc;
+    int have_Tsrc;
+    unsigned long long *norm;
+    int method; /* 0 Euler, 1 RK2, 2 RK4 */
+    long press_solves, steps;
+    double last_err;
+    int last_iters;
+    /* profiling */
+    int profiling;
+    hipEvent_t ev[2 * WF_MAX_EVENTS];
+    int ev_kind[WF_MAX_EVENTS];
+    int n_ev;
+    double k_ms[WF_K_COUNT];
+    long k_count[WF_K_COUNT];
+};
+
+static WfState wf_view(const wf_solver *s, double *base) {
+    WfState v;
+    v.u = base;
+    v.v = base + s->N;
+    v.w = base + 2 * s->N;
+    v.T = base + 3 * s->N;
+    v.Y = base + 4 * s->N;
+    return v;
+}
+
+static WfDerived wf_derived_view(const wf_solver *s) {
+    WfDerived d;
+    double *b = s->derived;
+    long N = s->N;
+    d.mu = b;
+    d.s11 = b + N;
+    d.s12 = b + 2 * N;
+    d.s13 = b + 3 * N;
+    d.s22 = b + 4 * N;
+    d.s23 = b + 5 * N;
+    d.s33 = b + 6 * N;
+    d.divU = b + 7 * N;
+    d.utau = b + 8 * N;
+    return d;
+}
+
+static int wf_factor(int n, int *rad) {
+    int cnt = 0;
+    while (n % 4 == 0 && cnt < WF_MAX_RADICES - 1) {
+        rad[cnt++] = 4;
+        n /= 4;
+    }
+    while (n % 2 == 0 && cnt < WF_MAX_RADICES - 1) {
+        rad[cnt++] = 2;
+        n /= 2;
+    }
+    for (int p = 3; n > 1 && cnt < WF_MAX_RADICES - 1; p += 2)
+        while (n % p == 0 && cnt < WF_MAX_RADICES - 1) {
+            rad[cnt++] = p;
+            n /= p;
+        }
+    if (n != 1) return -1;
+    for (int q = cnt; q < WF_MAX_RADICES; q++) rad[q] = 0;
+    return cnt;
+}
+
+template <typename T>
+static int wf_upload(T **dst, const T *src, size_t count) {
+    WF_HIP(hipMalloc((void **)dst, sizeof(T) * (count > 0 ? count : 1)));
+    if (count > 0) WF_HIP(hipMemcpy(*dst, src, sizeof(T) * count, hipMemcpyHostToDevice));
+    return WF_OK;
+}
+
+/* ------------------------------------------------------------ profiling ---- */
+
+static int wf_prof_begin(wf_solver *s, int kind) {
+    if (!s->profiling || s->n_ev >= WF_MAX_EVENTS) return -1;
+    int slot = s->n_ev++;
+    s->ev_kind[slot] = kind;
+    (void)hipEventRecord(s->ev[2 * slot], s->stream);
+    return slot;
+}
+
+static void wf_prof_end(wf_solver *s, int slot) {
+    if (slot >= 0) (void)hipEventRecord(s->ev[2 * slot + 1], s->stream);
+}
+
+static int wf_prof_collect(wf_solver *s) {
+    if (!s->profiling) return WF_OK;
+    WF_HIP(hipStreamSynchronize(s->stream));
+    for (int q = 0; q < s->n_ev; q++) {
+        float ms = 0.0f;
+        WF_HIP(hipEventElapsedTime(&ms, s->ev[2 * q], s->ev[2 * q + 1]));
+        s->k_ms[s->ev_kind[q]] += ms;
+        s->k_count[s->ev_kind[q]] += 1;
+    }
+    s->n_ev = 0;
+    return WF_OK;
+}
+
+#define WF_LAUNCH(solver, kind, expr)                                                        \
+    do {                                                                                     \
+        int slot_ = wf_prof_begin((solver), (kind));                                         \
+        hipError_t e_ = (expr);                                                              \
+        wf_prof_end((solver), slot_);                                                        \
+        if (e_ != hipSuccess)                                                                \
+            return wf_fail(WF_E_DEVICE, "launch %s: %s", #expr, hipGetErrorString(e_));     \
+    } while (0)
+
+/* ----------------------------------------------------------- lifecycle ---- */
+
+static int wf_validate(const Parameters *P) {
+    if (P->Nx < 5 || P->Ny < 5) return wf_fail(WF_E_ARG, "Nx and Ny must be at least 5 (got %d, %d)", P->Nx, P->Ny);
+    if (P->Nx - 1 > 1024 || P->Ny - 1 > 1024)
+        return wf_fail(WF_E_ARG, "Nx-1 and Ny-1 must not exceed 1024 (got %d, %d)", P->Nx - 1, P->Ny - 1);
+    if (P->Nz < 6) return wf_fail(WF_E_ARG, "Nz must be at least 6 (got %d)", P->Nz);
+    if (P->z == NULL || P->cut_nodes == NULL || P->Nz_Y == NULL || P->topography == NULL)
+        return wf_fail(WF_E_ARG, "parameters have no grids or terrain mask");
+    long cols = (long)P->Nx * P->Ny;
+    for (long q = 0; q < cols; q++)
+        if (P->cut_nodes[q] > P->Nz - 4 || P->cut_nodes[q] < 0)
+            return wf_fail(WF_E_ARG, "terrain reaches the top: cut node %d of column %ld exceeds Nz-4", P->cut_nodes[q], q);
+    return WF_OK;
+}
+
+static int wf_method(const Parameters *P) {
+    if (strncmp(P->method, "Euler", 5) == 0) return 0;
+    if (strncmp(P->method, "RK2", 3) == 0) return 1;
+    if (strncmp(P->method, "RK4", 3) == 0) return 2;
+    return -1;
+}
+
+static int wf_create_impl(wf_solver *s, const Parameters *P) {
+    int rc = wf_validate(P);
+    if (rc != WF_OK) return rc;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
+        return wf_fail(WF_E_DEVICE, "no HIP device available");
+    WF_HIP(hipSetDevice(0));
+    s->P = *P;
+    const int Nx = P->Nx, Ny = P->Ny, Nz = P->Nz, Nzm = P->Nz_Y_max;
+    s->plane = (long)Nx * Ny;
+    s->N = s->plane * Nz;
+    s->S = 4 * s->N + s->plane * Nzm;
+    s->method = wf_method(P);
+    WF_HIP(hipStreamCreate(&s->stream));
+    for (int q = 0; q < 4; q++) {
+        WF_HIP(hipMalloc((void **)&s->state[q], sizeof(double) * s->S));
+        WF_HIP(hipMemset(s->state[q], 0, sizeof(double) * s->S));
+    }
+    s->i_yn = 0;
+    s->i_x0 = 1;
+    s->i_x1 = 2;
+    s->i_acc = 3;
+    WF_HIP(hipMalloc((void **)&s->derived, sizeof(double) * (8 * s->N + s->plane)));
+    WF_HIP(hipMemset(s->derived, 0, sizeof(double) * (8 * s->N + s->plane)));
+    WF_HIP(hipMalloc((void **)&s->p, sizeof(double) * s->N));
+    WF_HIP(hipMemset(s->p, 0, sizeof(double) * s->N));
+    WF_HIP(hipMalloc((void **)&s->A, sizeof(double) * s->N));
+    WF_HIP(hipMalloc((void **)&s->scratch, sizeof(double) * s->S));
+    WF_HIP(hipMalloc((void **)&s->Tsrc, sizeof(double) * s->N));
+    WF_HIP(hipMalloc((void **)&s->norm, sizeof(unsigned long long)));
+
+    /* grid tables evaluated on the host exactly as the reference evaluates them */
+    const double *z = P->z;
+    std::vector<double> cd2(Nz);
+    for (int k = 0; k < Nz; k++) {
+        double dz_k = (k < Nz - 1) ? z[k + 1] - z[k] : z[k] - z[k - 1];
+        cd2[k] = pow(P->C_s * pow(P->dx * P->dy * dz_k, 1.0 / 3.0), 2.0);
+    }
+    if ((rc = wf_upload(&s->z, z, Nz))) return rc;
+    if ((rc = wf_upload(&s->cd2, cd2.data(), Nz))) return rc;
+    if ((rc = wf_upload(&s->topo, P->topography, s->plane))) return rc;
+    if ((rc = wf_upload(&s->x, P->x, Nx))) return rc;
+    if ((rc = wf_upload(&s->y, P->y, Ny))) return rc;
+    /* masks: the host arrays are indexed (i, j) with j fastest; the device
+     * tables are indexed col = i + Nx j */
+    std::vector<int> cut(s->plane), nzy(s->plane);
+    std::vector<double> topo(s->plane);
+    for (int i = 0; i < Nx; i++)
+        for (int j = 0; j < Ny; j++) {
+            cut[i + (long)Nx * j] = P->cut_nodes[(long)j + (long)Ny * i];
+            nzy[i + (long)Nx * j] = P->Nz_Y[(long)j + (long)Ny * i];
+            topo[i + (long)Nx * j] = P->topography[(long)j + (long)Ny * i];
+        }
+    WF_HIP(hipMemcpy(s->topo, topo.data(), sizeof(double) * s->plane, hipMemcpyHostToDevice));
+    if ((rc = wf_upload(&s->cut, cut.data(), s->plane))) return rc;
+    if ((rc = wf_upload(&s->nzy, nzy.data(), s->plane))) return rc;
+
+    WfConst &c = s->c;
+    memset(&c, 0, sizeof(c));
+    c.Nx = Nx;
+    c.Ny = Ny;
+    c.Nz = Nz;
+    c.Nzm = Nzm;
+    c.plane = s->plane;
+    c.N = s->N;
+    c.dx = P->dx;
+    c.dy = P->dy;
+    c.dt = P->dt;
+    c.mu = P->mu;
+    c.kT = P->k;
+    c.Y_f = P->Y_f;
+    c.H_C = P->H_C;
+    c.A = P->A;
+    c.T_act = P->T_act;
+    c.T_pc = P->T_pc;
+    c.h_c = P->h_c;
+    c.alpha_s = P->alpha_s;
+    c.sigma_s = P->sigma_s;
+    c.T_inf = P->T_inf;
+    c.c_p = P->c_p;
+    c.rho_inf = P->rho_inf;
+    c.C_d = P->C_d;
+    c.g = P->g;
+    c.delta = P->delta;
+    c.Pr = P->Pr;
+    c.T_min = P->T_min;
+    c.T_max = P->T_max;
+    c.Y_min = P->Y_min;
+    c.Y_max = P->Y_max;
+    c.u_dead = P->u_dead_nodes;
+    c.v_dead = P->v_dead_nodes;
+    c.w_dead = P->w_dead_nodes;
+    c.T_dead = P->T_dead_nodes;
+    c.Y_dead = P->Y_dead_nodes;
+    c.temperature_convection = P->temperature_convection;
+    c.suth_mu = P->sutherland_viscosity;
+    c.suth_k = P->sutherland_conductivity;
+    c.variable_density = P->variable_density;
+    c.fd_z = P->velocity_correction_fd;
+    c.exchange_on = (P->alpha_s * P->sigma_s != 0.0);
+    c.z = s->z;
+    c.cd2 = s->cd2;
+    c.topo = s->topo;
+    c.cut = s->cut;
+    c.nzy = s->nzy;
+
+    /* pressure solve tables (src/cu/pressure.cu:356-390) */
+    WfSpectral &sp = s->sp;
+    memset(&sp, 0, sizeof(sp));
+    sp.nx = Nx - 1;
+    sp.ny = Ny - 1;
+    sp.nz = Nz - 1;
+    sp.hx = sp.nx / 2 + 1;
+    sp.fx_pairs = (sp.nx <= 300) ? 4 : 2;
+    sp.fy_cols = (sp.ny <= 300) ? 4 : 2;
+    int radx[WF_MAX_RADICES], rady[WF_MAX_RADICES];
+    if (wf_factor(sp.nx, radx) < 0 || wf_factor(sp.ny, rady) < 0)
+        return wf_fail(WF_E_ARG, "transform length has too many factors");
+    if ((rc = wf_upload(&s->radx, radx, WF_MAX_RADICES))) return rc;
+    if ((rc = wf_upload(&s->rady, rady, WF_MAX_RADICES))) return rc;
+    std::vector<double2> twx(sp.nx), twy(sp.ny);
+    for (int e = 0; e < sp.nx; e++) {
+        double ang = 2.0 * M_PI * (double)e / (double)sp.nx;
+        twx[e] = make_double2(cos(ang), -sin(ang));
+    }
+    for (int e = 0; e < sp.ny; e++) {
+        double ang = 2.0 * M_PI * (double)e / (double)sp.ny;
+        twy[e] = make_double2(cos(ang), -sin(ang));
+    }
+    if ((rc = wf_upload(&s->twx, twx.data(), sp.nx))) return rc;
+    if ((rc = wf_upload(&s->twy, twy.data(), sp.ny))) return rc;
+    std::vector<double> kx2(sp.hx), ky2(sp.ny);
+    for (int r = 0; r < sp.hx; r++) kx2[r] = P->kx[r] * P->kx[r];
+    for (int q = 0; q < sp.ny; q++) ky2[q] = P->ky[q] * P->ky[q];
+    if ((rc = wf_upload(&s->kx2, kx2.data(), sp.hx))) return rc;
+    if ((rc = wf_upload(&s->ky2, ky2.data(), sp.ny))) return rc;
+    std::vector<double> ta(sp.nz, 0.0), tc(sp.nz, 0.0), beta(sp.nz, 0.0);
+    for (int k = 0; k < Nz - 2; k++) {
+        double dz_k = z[k + 1] - z[k], dz_kp1 = z[k + 2] - z[k + 1];
+        ta[k] = 2 / (dz_k * (dz_k + dz_kp1));
+        if (k > 0) {
+            double dz_km1 = z[k] - z[k - 1];
+            tc[k] = 2 / (dz_k * (dz_km1 + dz_k));
+            beta[k] = 2 / (dz_km1 * dz_k);
+        }
+    }
+    if ((rc = wf_upload(&s->ta, ta.data(), sp.nz))) return rc;
+    if ((rc = wf_upload(&s->tc, tc.data(), sp.nz))) return rc;
+    if ((rc = wf_upload(&s->beta, beta.data(), sp.nz))) return rc;
+    double dz0 = z[1] - z[0], dz1 = z[2] - z[1];
+    double dzNm2 = z[Nz - 2] - z[Nz - 3], dzNm1 = z[Nz - 1] - z[Nz - 2];
+    sp.dz0 = dz0;
+    sp.c0a = (dz0 + dz1) / (dz0 * dz1);
+    sp.c0b = 1 / dz1;
+    sp.b0 = -1.0 / dz0;
+    sp.beta_top = 2 / (dzNm2 * dzNm1);
+    sp.top_div = 2 / (dzNm1 * (dzNm2 + dzNm1));
+    long nspec = (long)sp.nz * sp.ny * sp.hx;
+    WF_HIP(hipMalloc((void **)&s->F, sizeof(double2) * nspec));
+    WF_HIP(hipMemset(s->F, 0, sizeof(double2) * nspec));
+    WF_HIP(hipMalloc((void **)&s->Ptop, sizeof(double2) * (long)sp.ny * sp.hx));
+    WF_HIP(hipMemset(s->Ptop, 0, sizeof(double2) * (long)sp.ny * sp.hx));
+    WF_HIP(hipMalloc((void **)&s->U, sizeof(double) * nspec));
+    sp.F = s->F;
+    sp.Ptop = s->Ptop;
+    sp.A = s->A;
+    sp.U = s->U;
+    sp.kx2 = s->kx2;
+    sp.ky2 = s->ky2;
+    sp.ta = s->ta;
+    sp.tc = s->tc;
+    sp.beta = s->beta;
+    sp.twx = s->twx;
+    sp.twy = s->twy;
+    sp.radx = s->radx;
+    sp.rady = s->rady;
+    sp.norm_bits = s->norm;
+    WF_HIP(wf_launch_pivots(sp, s->U, s->stream));
+    WF_HIP(hipStreamSynchronize(s->stream));
+    for (int q = 0; q < 2 * WF_MAX_EVENTS; q++) WF_HIP(hipEventCreate(&s->ev[q]));
+    return WF_OK;
+}
+
+extern "C" int wf_solver_create(const Parameters *parameters, wf_solver **out) {
+    if (parameters == NULL || out == NULL) return wf_fail(WF_E_ARG, "null argument");
+    wf_solver *s = (wf_solver *)calloc(1, sizeof(wf_solver));
+    if (s == NULL) return wf_fail(WF_E_ALLOC, "out of host memory");
+    int rc = wf_create_impl(s, parameters);
+    if (rc != WF_OK) {
+        char keep[512];
+        memcpy(keep, wf_err, sizeof(keep));
+        wf_solver_destroy(s);
+        memcpy(wf_err, keep, sizeof(keep));
+        *out = NULL;
+        return rc;
+    }
+    *out = s;
+    return WF_OK;
+}
+
+extern "C" int wf_solver_destroy(wf_solver *s) {
+    if (s == NULL) return WF_OK;
+    if (s->stream) (void)hipStreamSynchronize(s->stream);
+    for (int q = 0; q < 4; q++) (void)hipFree(s->state[q]);
+    void *bufs[] = {s->derived, s->p, s->A, s->U, s->F, s->Ptop, s->z, s->cd2, s->topo, s->kx2, s->ky2, s->ta,
+                    s->tc, s->beta, s->x, s->y, s->cut, s->nzy, s->radx, s->rady, s->twx, s->twy, s->scratch,
+                    s->Tsrc, s->norm};
+    for (size_t q = 0; q < sizeof(bufs) / sizeof(bufs[0]); q++)
+        if (bufs[q]) (void)hipFree(bufs[q]);
+    for (int q = 0; q < 2 * WF_MAX_EVENTS; q++)
+        if (s->ev[q]) (void)hipEventDestroy(s->ev[q]);
+    if (s->stream) (void)hipStreamDestroy(s->stream);
+    free(s);
+    return WF_OK;
+}
+
+/* ------------------------------------------------------ layout transfers ---- */
+
+/* host reference-layout state vector -> device internal-layout buffer */
+static int wf_put_state(wf_solver *s, const double *y_host, double *dst) {
+    const Parameters &P = s->P;
+    WF_HIP(hipMemcpyAsync(s->scratch, y_host, sizeof(double) * s->S, hipMemcpyHostToDevice, s->stream));
+    for (int f = 0; f < 4; f++)
+        WF_HIP(wf_launch_to_internal(s->scratch + f * s->N, dst + f * s->N, P.Nx, P.Ny, P.Nz, s->stream));
+    WF_HIP(wf_launch_to_internal(s->scratch + 4 * s->N, dst + 4 * s->N, P.Nx, P.Ny, P.Nz_Y_max, s->stream));
+    return WF_OK;
+}
+
+static int wf_get_state_buf(wf_solver *s, const double *src, double *y_host) {
+    const Parameters &P = s->P;
+    for (int f = 0; f < 4; f++)
+        WF_HIP(wf_launch_to_reference(src + f * s->N, s->scratch + f * s->N, P.Nx, P.Ny, P.Nz, s->stream));
+    WF_HIP(wf_launch_to_reference(src + 4 * s->N, s->scratch + 4 * s->N, P.Nx, P.Ny, P.Nz_Y_max, s->stream));
+    WF_HIP(hipMemcpyAsync(y_host, s->scratch, sizeof(double) * s->S, hipMemcpyDeviceToHost, s->stream));
+    WF_HIP(hipStreamSynchronize(s->stream));
+    return WF_OK;
+}
+
+static int wf_put_field(wf_solver *s, const double *f_host, double *dst) {
+    const Parameters &P = s->P;
+    WF_HIP(hipMemcpyAsync(s->scratch, f_host, sizeof(double) * s->N, hipMemcpyHostToDevice, s->stream));
+    WF_HIP(wf_launch_to_internal(s->scratch, dst, P.Nx, P.Ny, P.Nz, s->stream));
+    return WF_OK;
+}
+
+static int wf_get_field(wf_solver *s, const double *src, double *f_host) {
+    const Parameters &P = s->P;
+    WF_HIP(wf_launch_to_reference(src, s->scratch, P.Nx, P.Ny, P.Nz, s->stream));
+    WF_HIP(hipMemcpyAsync(f_host, s->scratch, sizeof(double) * s->N, hipMemcpyDeviceToHost, s->stream));
+    WF_HIP(hipStreamSynchronize(s->stream));
+    return WF_OK;
+}
+
+/* spectrum of the top pressure plane (fixed for the whole run: the solve never
+ * writes the unique nodes of k = Nz-1) */
+static int wf_top_spectrum(wf_solver *s) {
+    WF_HIP(wf_launch_ptop_fx(s->c, s->sp, s->p, s->stream));
+    WF_HIP(wf_launch_fy(s->sp, s->Ptop, 0, 1, -1, s->stream));
+    return WF_OK;
+}
+
+extern "C" int wf_solver_set_state(wf_solver *s, const double *y_n, const double *p) {
+    if (s == NULL || y_n == NULL || p == NULL) return wf_fail(WF_E_ARG, "null argument");
+    int rc;
+    if ((rc = wf_put_state(s, y_n, s->state[s->i_yn]))) return rc;
+    if ((rc = wf_put_field(s, p, s->p))) return rc;
+    if (!s->have_Tsrc) {
+        WF_HIP(hipMemcpyAsync(s->Tsrc, s->state[s->i_yn] + 3 * s->N, sizeof(double) * s->N, hipMemcpyDeviceToDevice,
+                              s->stream));
+        s->have_Tsrc = 1;
+    }
+    if ((rc = wf_top_spectrum(s))) return rc;
+    WF_HIP(hipStreamSynchronize(s->stream));
+    return WF_OK;
+}
+
+extern "C" int wf_solver_get_state(wf_solver *s, double *y_n, double *p) {
+    if (s == NULL) return wf_fail(WF_E_ARG, "null argument");
+    int rc;
+    if (y_n && (rc = wf_get_state_buf(s, s->state[s->i_yn], y_n))) return rc;
+    if (p && (rc = wf_get_field(s, s->p, p))) return rc;
+    return WF_OK;
+}
+
+/* --------------------------------------------------------------- pieces ---- */
+
+/* Phi of one stage: derived pass on X, then the stage pass in `mode`. */
+static int wf_stage(wf_solver *s, double *X, double *out, int mode, double a) {
+    WfState xs = wf_view(s, X), yn = wf_view(s, s->state[s->i_yn]), acc = wf_view(s, s->state[s->i_acc]);
+    WfState o = wf_view(s, out);
+    WfDerived D = wf_derived_view(s);
+    WF_LAUNCH(s, WF_K_DERIVED, wf_launch_derived(s->c, xs, D, s->stream));
+    WF_LAUNCH(s, WF_K_STAGE, wf_launch_stage(s->c, xs, D, yn, acc, o, mode, a, s->stream));
+    return WF_OK;
+}
+
+/* RK stages of src/cu/solver.cu:59-93: y_n -> y* in state[i_x1]. */
+static int wf_integrate(wf_solver *s) {
+    double dt = s->P.dt;
+    double *yn = s->state[s->i_yn], *x0 = s->state[s->i_x0], *x1 = s->state[s->i_x1];
+    int rc;
+    if (s->method == 2) {
+        if ((rc = wf_stage(s, yn, x0, WF_MODE_FIRST, dt * 0.5))) return rc;
+        if ((rc = wf_stage(s, x0, x1, WF_MODE_MID, dt * 0.5))) return rc;
+        if ((rc = wf_stage(s, x1, x0, WF_MODE_MID, dt))) return rc;
+        if ((rc = wf_stage(s, x0, x1, WF_MODE_LAST, dt / 6))) return rc;
+    } else if (s->method == 1) {
+        if ((rc = wf_stage(s, yn, x0, WF_MODE_FIRST, dt))) return rc;
+        if ((rc = wf_stage(s, x0, x1, WF_MODE_LAST, 0.5 * dt))) return rc;
+    } else if (s->method == 0) {
+        if ((rc = wf_stage(s, yn, x1, WF_MODE_EULER, dt))) return rc;
+    } else {
+        return wf_fail(WF_E_METHOD, "Time integration method not found.");
+    }
+    return WF_OK;
+}
+
+/* Pressure solve of src/cu/pressure.cu:506-606 for y* = Ys, y_n = Yn. */
+static int wf_pressure_solve(wf_solver *s, double *Ys_buf, double *Yn_buf) {
+    const Parameters &P = s->P;
+    WfState ys = wf_view(s, Ys_buf), yn = wf_view(s, Yn_buf);
+    int iters = (P.variable_density == 0) ? 1 : P.pressure_solver_iter;
+    double err = 0.0;
+    int m;
+    for (m = 0; m < iters; m++) {
+        WF_HIP(hipMemsetAsync(s->norm, 0, sizeof(unsigned long long), s->stream));
+        WF_LAUNCH(s, WF_K_PRHS, wf_launch_pressure_rhs_fx(s->c, s->sp, ys, yn, s->p, m == 0, s->stream));
+        WF_LAUNCH(s, WF_K_PFY, wf_launch_fy(s->sp, s->F, 1, s->sp.nz - 1, -1, s->stream));
+        WF_LAUNCH(s, WF_K_THOMAS, wf_launch_thomas(s->sp, s->stream));
+        WF_LAUNCH(s, WF_K_PFY, wf_launch_fy(s->sp, s->F, 0, s->sp.nz, +1, s->stream));
+        WF_LAUNCH(s, WF_K_PINV, wf_launch_inverse_fx(s->c, s->sp, s->p, s->stream));
+        WF_LAUNCH(s, WF_K_OTHER, wf_launch_top_duplicates(s->c, s->sp, s->p, s->stream));
+        s->press_solves++;
+        if (P.variable_density == 0) break;
+        unsigned long long bits = 0;
+        WF_HIP(hipMemcpyAsync(&bits, s->norm, sizeof(bits), hipMemcpyDeviceToHost, s->stream));
+        WF_HIP(hipStreamSynchronize(s->stream));
+        memcpy(&err, &bits, sizeof(err));
+        if (err <= P.pressure_solver_tol) break;
+    }
+    s->last_err = err;
+    s->last_iters = m;
+    return WF_OK;
+}
+
+static WfSource wf_source(wf_solver *s, int n) {
+    const Parameters &P = s->P;
+    WfSource src;
+    memset(&src, 0, sizeof(src));
+    double t_n = P.t[n];
+    if (!(t_n <= P.t_source_end)) return src;
+    if (P.t_source_start <= 0) {
+        src.mode = s->have_Tsrc ? 1 : 0;
+        src.T_src = s->Tsrc;
+        return src;
+    }
+    src.mode = 2;
+    src.shape = (strcmp(P.T_source_shape, "gaussian") == 0) ? 1 : (strcmp(P.T_source_shape, "parallelepiped") == 0 ? 2 : 0);
+    src.scale = (t_n <= P.t_source_start) ? t_n / P.t_source_start : 1.0;
+    src.T_inf = P.T_inf;
+    src.T_source = P.T_source;
+    src.xc = P.TS_x_center;
+    src.yc = P.TS_y_center;
+    src.zc = P.TS_z_center;
+    src.sx = P.TS_length;
+    src.sy = P.TS_width;
+    src.sz = P.TS_height;
+    src.x0 = P.TS_x_start;
+    src.x1 = P.TS_x_end;
+    src.y0 = P.TS_y_start;
+    src.y1 = P.TS_y_end;
+    src.z0 = P.TS_z_start;
+    src.z1 = P.TS_z_end;
+    src.x = s->x;
+    src.y = s->y;
+    return src;
+}
+
+extern "C" int wf_solver_step(wf_solver *s, int step0, int n_steps) {
+    if (s == NULL) return wf_fail(WF_E_ARG, "null solver");
+    if (s->method < 0) return wf_fail(WF_E_METHOD, "Time integration method not found.");
+    if (step0 < 1 || step0 + n_steps - 1 > s->P.Nt)
+        return wf_fail(WF_E_ARG, "steps %d..%d outside 1..Nt=%d", step0, step0 + n_steps - 1, s->P.Nt);
+    int rc;
+    for (int n = step0; n < step0 + n_steps; n++) {
+        if ((rc = wf_integrate(s))) return rc;
+        double *ys = s->state[s->i_x1], *yn = s->state[s->i_yn], *out = s->state[s->i_x0];
+        if ((rc = wf_pressure_solve(s, ys, yn))) return rc;
+        WfSource src = wf_source(s, n);
+        WF_LAUNCH(s, WF_K_PROJECT,
+                  wf_launch_project(s->c, wf_view(s, ys), wf_view(s, yn), s->p, wf_view(s, out), src, s->stream));
+        /* y_n <- y_{n+1} by rotating buffer roles */
+        int old_yn = s->i_yn;
+        s->i_yn = s->i_x0;
+        s->i_x0 = old_yn;
+        s->steps++;
+        if ((rc = wf_prof_collect(s))) return rc;
+    }
+    WF_HIP(hipStreamSynchronize(s->stream));
+    return WF_OK;
+}
+
+extern "C" int wf_solver_phi(wf_solver *s, const double *R_old, double *R_new) {
+    if (s == NULL || R_old == NULL || R_new == NULL) return wf_fail(WF_E_ARG, "null argument");
+    int rc;
+    double *x0 = s->state[s->i_x0], *x1 = s->state[s->i_x1];
+    if ((rc = wf_put_state(s, R_old, x0))) return rc;
+    if ((rc = wf_stage(s, x0, x1, WF_MODE_PHI, 0.0))) return rc;
+    if ((rc = wf_get_state_buf(s, x1, R_new))) return rc;
+    return wf_prof_collect(s);
+}
+
+extern "C" int wf_solver_pressure(wf_solver *s, const double *y_np1, const double *y_n, double *p, double *error,
+                                  int *iterations) {
+    if (s == NULL || y_np1 == NULL || y_n == NULL || p == NULL) return wf_fail(WF_E_ARG, "null argument");
+    int rc;
+    double *x1 = s->state[s->i_x1], *yn = s->state[s->i_yn];
+    if ((rc = wf_put_state(s, y_np1, x1))) return rc;
+    if ((rc = wf_put_state(s, y_n, yn))) return rc;
+    if ((rc = wf_put_field(s, p, s->p))) return rc;
+    if ((rc = wf_top_spectrum(s))) return rc;
+    if ((rc = wf_pressure_solve(s, x1, yn))) return rc;
+    if ((rc = wf_get_field(s, s->p, p))) return rc;
+    if (error) *error = s->last_err;
+    if (iterations) *iterations = s->last_iters;
+    return wf_prof_collect(s);
+}
+
+extern "C" int wf_solver_project(wf_solver *s, double *y_np1, const double *y_n, const double *p, int n) {
+    if (s == NULL || y_np1 == NULL || y_n == NULL || p == NULL) return wf_fail(WF_E_ARG, "null argument");
+    if (n < 0 || n > s->P.Nt) return wf_fail(WF_E_ARG, "step %d outside 0..Nt", n);
+    int rc;
+    double *x1 = s->state[s->i_x1], *yn = s->state[s->i_yn], *x0 = s->state[s->i_x0];
+    if ((rc = wf_put_state(s, y_np1, x1))) return rc;
+    if ((rc = wf_put_state(s, y_n, yn))) return rc;
+    if ((rc = wf_put_field(s, p, s->p))) return rc;
+    WfSource src = wf_source(s, n);
+    WF_LAUNCH(s, WF_K_PROJECT,
+              wf_launch_project(s->c, wf_view(s, x1), wf_view(s, yn), s->p, wf_view(s, x0), src, s->stream));
+    if ((rc = wf_get_state_buf(s, x0, y_np1))) return rc;
+    return wf_prof_collect(s);
+}
+
+extern "C" int wf_solver_stats(const wf_solver *s, long *pressure_iterations, long *steps, double *last_error) {
+    if (s == NULL) return wf_fail(WF_E_ARG, "null solver");
+    if (pressure_iterations) *pressure_iterations = s->press_solves;
+    if (steps) *steps = s->steps;
+    if (last_error) *last_error = s->last_err;
+    return WF_OK;
+}
+
+extern "C" int wf_solver_profile(wf_solver *s, int enable) {
+    if (s == NULL) return wf_fail(WF_E_ARG, "null solver");
+    s->profiling = enable ? 1 : 0;
+    s->n_ev = 0;
+    for (int q = 0; q < WF_K_COUNT; q++) {
+        s->k_ms[q] = 0.0;
+        s->k_count[q] = 0;
+    }
+    return WF_OK;
+}
+
+extern "C" int wf_solver_kernel_times(const wf_solver *s, int kind, double *total_ms, long *launches) {
+    if (s == NULL || kind < 0 || kind >= WF_K_COUNT) return wf_fail(WF_E_ARG, "bad argument");
+    if (total_ms) *total_ms = s->k_ms[kind];
+    if (launches) *launches = s->k_count[kind];
+    return WF_OK;
+}
+
+extern "C" int wf_device_synchronize(void) {
+    WF_HIP(hipDeviceSynchronize());
+    return WF_OK;
+}
+
+/* last pressure-iteration count and error, for the step-loop driver */
+extern "C" int wf_solver_last_pressure(const wf_solver *s, double *error, int *iterations) {
+    if (s == NULL) return wf_fail(WF_E_ARG, "null solver");
+    if (error) *error = s->last_err;
+    if (iterations) *iterations = s->last_iters;
+    return WF_OK;
+}

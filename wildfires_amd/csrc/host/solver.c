@@ -1,0 +1,78 @@
+/*
+ * solver.c -- solve_PDE: the time loop of src/cu/solver.cu:95-298 driven
+ * through the wf_* entry points, with the reference's timing lines,
+ * checkpoint reports, logs and snapshot files.
+ */
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <sys/time.h>
+
+#include "wf_host.h"
+
+static double wf_elapsed(const struct timeval *a, const struct timeval *b) {
+    return ((b->tv_sec - a->tv_sec) * 1000000u + b->tv_usec - a->tv_usec) / 1.e6;
+}
+
+static void wf_fatal(Parameters parameters, const char *what) {
+    char msg[768];
+    snprintf(msg, sizeof(msg), "%s: %s", what, wf_last_error());
+    log_message(parameters, msg);
+    exit(1);
+}
+
+void solve_PDE(double *y_n, double *p, Parameters parameters) {
+    int Nt = parameters.Nt, NT = parameters.NT;
+    long size = wf_state_size(&parameters);
+    long n3 = (long)parameters.Nx * parameters.Ny * parameters.Nz;
+    double *t = parameters.t;
+    double CFL = 0.0, T_min = 1e9, T_max = -1e9, Y_min = 1e9, Y_max = -1e9;
+    double cum_step_time = 0.0, step_time = 0.0, error = 0.0;
+    int iterations = 0;
+    char message[256], formatted[64];
+    struct timeval start_solver, end_solver, start_ts, end_ts;
+    wf_solver *solver = NULL;
+    if (wf_solver_create(&parameters, &solver) != WF_OK) wf_fatal(parameters, "Device setup failed");
+    if (wf_solver_set_state(solver, y_n, p) != WF_OK) wf_fatal(parameters, "Device state upload failed");
+    double *y_host = (double *)malloc(sizeof(double) * (size_t)size);
+    double *p_host = (double *)malloc(sizeof(double) * (size_t)n3);
+    gettimeofday(&start_solver, NULL);
+    for (int n = 1; n <= Nt; n++) {
+        gettimeofday(&start_ts, NULL);
+        int rc = wf_solver_step(solver, n, 1);
+        if (rc == WF_E_METHOD) {
+            log_message(parameters, "Time integration method not found.");
+            exit(1);
+        }
+        if (rc != WF_OK) wf_fatal(parameters, "Time step failed");
+        wf_solver_last_pressure(solver, &error, &iterations);
+        if (parameters.variable_density == 1 && parameters.pressure_solver_log == 1) {
+            sprintf(message, "Pressure solver: Error = %e, iterations = %d", error, iterations);
+            log_message(parameters, message);
+        }
+        gettimeofday(&end_ts, NULL);
+        step_time = wf_elapsed(&start_ts, &end_ts);
+        cum_step_time += step_time;
+        if (parameters.time_step_log > 0 && n % parameters.time_step_log == 0)
+            printf("Time step: %d, Average time: %lf s\n", n, cum_step_time / n);
+        if (n % NT == 0 || n == Nt) {
+            if (wf_solver_get_state(solver, y_host, p_host) != WF_OK) wf_fatal(parameters, "Device state download failed");
+            timestep_reports(y_host, &CFL, &Y_min, &Y_max, &T_min, &T_max, n, parameters);
+            log_timestep(parameters, n, t[n], step_time, CFL, T_min, T_max, Y_min, Y_max);
+            if (parameters.variable_density == 1) {
+                sprintf(message, "Pressure solver: Error = %e, iterations = %d", error, iterations);
+                log_message(parameters, message);
+            }
+            save_data(y_host, p_host, n / NT, parameters);
+            printf("\n");
+        }
+    }
+    gettimeofday(&end_solver, NULL);
+    format_seconds(wf_elapsed(&start_solver, &end_solver), formatted);
+    sprintf(message, "\nSolver time: %s", formatted);
+    log_message(parameters, message);
+    if (wf_solver_get_state(solver, y_n, p) != WF_OK) wf_fatal(parameters, "Device state download failed");
+    free(y_host);
+    free(p_host);
+    wf_solver_destroy(solver);
+}
