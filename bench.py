@@ -1,0 +1,197 @@
+"""Throughput of the FP64 time step on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload hill256]
+
+Workload (default): the 256x256x128 hill run (400x400x50 m, Gaussian hill,
+seeded fuel layer, line ignition, RK4, dt = 0.005 s), state resident in HBM.
+A step is one complete time step: four RK stages, the variable-density
+pressure iteration (all its solves), projection, boundary conditions, bounds.
+value = cells x steps / wall seconds over K steps after W warm-up steps,
+bracketed by device synchronisation.
+
+Also reported:
+  roofline      the dominant kernel's algorithmic bytes per launch over its mean
+                duration, timed live with HIP events on the solver's stream in a
+                separately profiled run of the same steps;
+  step_roofline the whole-step algorithmic traffic 8 B x (92 + 19 phi_Y +
+                15 (n_iter - 1)) per cell-step (phi_Y = Nz_Y_max/Nz) at the
+                measured rate, against 8 TB/s;
+  cpu_baseline  the FP64 CPU oracle (OpenMP, all host cores of this process)
+                on one step of the same workload.
+"""
+import argparse
+import json
+import os
+import sys
+import tempfile
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def kernel_bytes(params, stats):
+    """Algorithmic bytes per launch of each kernel class: every array the
+    kernel needs is read once and every array it produces is written once."""
+    P = params.p
+    N = P.Nx * P.Ny * P.Nz
+    plane = P.Nx * P.Ny
+    NY = plane * P.Nz_Y_max
+    S = 4 * N + NY
+    nx, ny, nz = P.Nx - 1, P.Ny - 1, P.Nz - 1
+    hx = nx // 2 + 1
+    spec = 16.0 * hx * ny * nz      # one complex half-spectrum sweep
+    d = 8.0
+    out = {}
+    # derived pass: reads u,v,w,T (+Y on fuel nodes), writes 8 fields + u_tau plane
+    out["derived"] = d * (S + 8 * N + plane)
+    # stage pass, mean over the RK4 modes FIRST/MID/MID/LAST: stage input S,
+    # derived 8N, y_n S, acc (3 of 4 stages) S, out S, acc (3 of 4) S
+    out["stage"] = d * (S + 8 * N + S + 0.75 * S + S + 0.75 * S)
+    it = max(stats.get("solves_per_step", 1.0), 1.0)
+    # P1: first solve of a step reads y* (u,v,w,T,Y), T^n, p and writes A;
+    # later solves read A, T^n, p; all write one half spectrum
+    first = d * (4 * N + NY + N + N + N) + spec
+    later = d * 3 * N + spec
+    out["pressure_rhs_fx"] = (first + (it - 1) * later) / it
+    out["pressure_fy"] = 2 * spec
+    out["thomas"] = 2 * spec + 2 * spec + 2 * 8.0 * hx * ny * nz   # d, y (w+r), x, pivots twice
+    out["pressure_inverse_fx"] = spec + 2 * d * N
+    out["project"] = d * (S + N + N + S)
+    return out
+
+
+def run(args):
+    import numpy as np
+    from wildfires_amd import api, configs
+
+    wd = tempfile.mkdtemp(prefix="wf_bench_")
+    path = os.path.join(wd, args.workload + ".txt")
+    total = args.warmup + args.steps + args.profile_steps
+    configs.write_parameter_file(path, args.workload, Nt=max(total, 2), NT=max(total, 2),
+                                 t_max=0.005 * max(total, 2) if args.workload == "hill256" else 0.01 * max(total, 2))
+    params = api.read_parameters(path, output_root=os.path.join(wd, "out"))
+    P = params.p
+    if args.workload == "hill256":
+        y0, p0 = configs.hill256_initial_state(params)
+    else:
+        y0, p0 = api.initial_conditions(params)
+    N = P.Nx * P.Ny * P.Nz
+    log("workload %s: %dx%dx%d, Nz_Y_max %d, dt %g" % (args.workload, P.Nx, P.Ny, P.Nz, P.Nz_Y_max, P.dt))
+    solver = api.Solver(params)
+    solver.set_state(y0, p0)
+    n = 1
+    if args.warmup:
+        solver.step(n, args.warmup)
+        n += args.warmup
+    log("warm-up done (%d steps)" % args.warmup)
+    s0 = solver.stats()
+    api.synchronize()
+    t0 = time.perf_counter()
+    solver.step(n, args.steps)
+    api.synchronize()
+    t1 = time.perf_counter()
+    n += args.steps
+    s1 = solver.stats()
+    elapsed = t1 - t0
+    rate = N * args.steps / elapsed / 1e6
+    solves = (s1["pressure_solves"] - s0["pressure_solves"]) / args.steps
+    log("timed %d steps in %.3f s: %.1f Mcell-steps/s, %.1f pressure solves/step" % (args.steps, elapsed, rate, solves))
+
+    # per-kernel timing (HIP events on the solver stream) over profile_steps more steps
+    solver.profile(True)
+    solver.step(n, args.profile_steps)
+    times = solver.kernel_times()
+    solver.profile(False)
+    kbytes = kernel_bytes(params, {"solves_per_step": solves})
+    kernels = {}
+    for name, (ms, cnt) in times.items():
+        if cnt == 0:
+            continue
+        entry = {"total_ms": round(ms, 4), "launches": cnt, "mean_us": round(1000.0 * ms / cnt, 2)}
+        if name in kbytes:
+            entry["algorithmic_MB"] = round(kbytes[name] / 1e6, 3)
+            entry["GBs"] = round(kbytes[name] / (ms / cnt / 1e3) / 1e9, 1)
+        kernels[name] = entry
+    dom = max((k for k in kernels if k in kbytes), key=lambda k: kernels[k]["total_ms"])
+    dom_mean_s = kernels[dom]["total_ms"] / kernels[dom]["launches"] / 1e3
+    achieved = kbytes[dom] / dom_mean_s / 1e9
+    y_state, _ = solver.get_state()
+    finite = bool(np.isfinite(y_state).all())
+    phi = P.Nz_Y_max / P.Nz
+    b_step = 8.0 * (92 + 19 * phi + 15 * (solves - 1))
+    step_achieved = b_step * rate * 1e6 / 1e9
+    result = {
+        "metric": "Mcell-steps/s at 256x256x128 (plain FP64 time step; % HBM peak)",
+        "value": round(rate, 3),
+        "unit": "Mcell-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1000.0 * elapsed / args.steps, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "hill256: 256x256x128 cells, 400x400x50 m, Gaussian hill 20 m, seeded fuel "
+                               "layer 0.5 m, line ignition, RK4, dt=0.005 s, variable density",
+                   "cells": N, "pressure_solves_per_step": round(solves, 3), "state_finite": finite},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "algorithmic_bytes_per_launch": int(kbytes[dom]),
+                     "mean_launch_us": round(dom_mean_s * 1e6, 2)},
+        "step_roofline": {"bytes_per_cell_step": round(b_step, 1), "achieved": round(step_achieved, 1),
+                          "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(step_achieved / HBM_PEAK_GBS, 4)},
+        "kernels": kernels,
+    }
+    solver.close()
+    if not args.no_cpu_baseline:
+        result["cpu_baseline"] = cpu_baseline(params, y0, p0)
+    return result
+
+
+def cpu_baseline(params, y0, p0):
+    """FP64 CPU oracle (C/OpenMP) on one step of the same workload."""
+    from oracle import oracle as orc
+    ncores = len(os.sched_getaffinity(0))
+    O = orc.Oracle(params.ptr)
+    y, p = y0.copy(), p0.copy()
+    t0 = time.perf_counter()
+    O.step(y, p, 1)
+    dt = time.perf_counter() - t0
+    solves = O.pressure_solves()
+    N = params.p.Nx * params.p.Ny * params.p.Nz
+    threads = orc.threads()
+    log("cpu oracle: 1 step in %.2f s on %d threads (%d of %d cores in this process's affinity)"
+        % (dt, threads, threads, ncores))
+    O.close()
+    return {"value": round(N / dt / 1e6, 4), "unit": "Mcell-steps/s", "cores": threads, "kind": "port",
+            "sample": "1 full step (RK4 + %d pressure solves) of the same %dx%dx%d workload from its initial state"
+                      % (solves, params.p.Nx, params.p.Ny, params.p.Nz)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--profile-steps", type=int, default=2)
+    ap.add_argument("--workload", default="hill256", choices=["hill256", "flat64"])
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (every box has one GPU)",
+              file=sys.stderr)
+        sys.exit(2)
+    print(json.dumps(run(args)), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -33,11 +33,18 @@ def workdir(tmp_path, monkeypatch):
     return tmp_path
 
 
+def short_dir():
+    """A fresh directory with a short absolute path: the run's output directory
+    must fit the 64-byte save_path of the Parameters structure."""
+    import tempfile
+    return tempfile.mkdtemp(prefix="wf", dir="/tmp")
+
+
 def make_params(tmp_path, name, **over):
     from wildfires_amd import configs, api
     path = os.path.join(str(tmp_path), name + ".txt")
     configs.write_parameter_file(path, name, **over)
-    return api.read_parameters(path, output_root=os.path.join(str(tmp_path), "out"))
+    return api.read_parameters(path, output_root=short_dir())
 
 
 def rel_l2(a, b):

@@ -68,6 +68,16 @@ hipError_t wf_launch_to_internal(const double *src, double *dst, int Nx, int Ny,
 hipError_t wf_launch_to_reference(const double *src, double *dst, int Nx, int Ny, int nz, hipStream_t s);
 
 /* pressure pass (wf_pressure.hip) */
+
+/* Device-side control of the variable-density pressure iteration. */
+struct WfIterCtl {
+    unsigned long long norm_bits; /* max |p_new - p_old| of the current solve, as ordered bits */
+    int done;                     /* 1 once converged or out of iterations */
+    int iters;                    /* the count the reference reports */
+    int solves;                   /* solves performed */
+    double err;                   /* last max |p_new - p_old| */
+};
+
 struct WfSpectral {
     int nx, ny, nz; /* unique nodes per direction: Nx-1, Ny-1, Nz-1 */
     int hx;         /* nx/2 + 1 retained x-wavenumbers */
@@ -89,17 +99,21 @@ struct WfSpectral {
     double b0;                     /* b_0 = -1/dz0 */
     double beta_top;               /* 2/(dz_{Nz-2} dz_{Nz-1}) */
     const double *beta;            /* nz: 2/(dz_{k-1} dz_k) for 1 <= k <= nz-2 */
-    unsigned long long *norm_bits; /* max |p_new - p_old| as ordered bits */
+    const double *rho;             /* rho^n on every node (variable density) */
+    WfIterCtl *ctl;
 };
 
 hipError_t wf_launch_pressure_rhs_fx(const WfConst &c, const WfSpectral &sp, const WfState &Ys, const WfState &Yn,
                                      const double *p, int first_iter, hipStream_t s);
 hipError_t wf_launch_ptop_fx(const WfConst &c, const WfSpectral &sp, const double *p, hipStream_t s);
-hipError_t wf_launch_fy(const WfSpectral &sp, double2 *buf, int slab0, int nslabs, int sign, hipStream_t s);
+hipError_t wf_launch_fy(const WfSpectral &sp, double2 *buf, int slab0, int nslabs, int sign, int guarded,
+                        hipStream_t s);
+hipError_t wf_launch_density(const WfConst &c, const double *T, double *rho, hipStream_t s);
+hipError_t wf_launch_iter_reset(WfIterCtl *ctl, hipStream_t s);
+hipError_t wf_launch_iter_check(WfIterCtl *ctl, double tol, int m, int max_iter, hipStream_t s);
 hipError_t wf_launch_thomas(const WfSpectral &sp, hipStream_t s);
 hipError_t wf_launch_pivots(const WfSpectral &sp, double *U, hipStream_t s);
 hipError_t wf_launch_inverse_fx(const WfConst &c, const WfSpectral &sp, double *p, hipStream_t s);
-hipError_t wf_launch_top_duplicates(const WfConst &c, const WfSpectral &sp, double *p, hipStream_t s);
 
 /* Temperature source applied at the end of a step (src/cu/functions.cu:325-403). */
 struct WfSource {

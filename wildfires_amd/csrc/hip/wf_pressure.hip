@@ -13,15 +13,18 @@
  *   P2 k_fy       : y-transform of the half spectrum (forward/inverse) [per slab, kx block]
  *   P3 k_thomas   : tridiagonal solve per (kx, ky), pivots precomputed [per system]
  *   P5 k_inv_fx   : inverse x-transform, p, periodic duplicates, |dp|max [per row pair]
+ *
+ * The variable-density fixed-point loop is controlled on the device
+ * (WfIterCtl): every kernel of a solve returns at once after convergence, so
+ * the host queues solves without waiting for each norm.
  */
 #include "wf_common.h"
 #include "wf_fft.h"
 #include "wf_kernels.h"
 
-#define WF_MAX_N 1024   /* longest supported transform (LDS is sized per launch) */
-
-/* One full FFT of `nseq` sequences of length n held in LDS (a -> b -> a ...).
- * Returns the buffer that holds the result. */
+/* One full FFT of `nseq` sequences of run-time length n held in LDS
+ * (a -> b -> a ...), for lengths without a compile-time plan.  Returns the
+ * buffer that holds the result. */
 __device__ wfc *wf_lds_fft(wfc *a, wfc *b, int n, int nseq, const int *rad, const wfc *tw) {
     int Ns = 1;
     for (int p = 0; p < WF_MAX_RADICES; p++) {
@@ -40,8 +43,8 @@ __device__ wfc *wf_lds_fft(wfc *a, wfc *b, int n, int nseq, const int *rad, cons
     return a;
 }
 
-/* p-independent part of f at node (i,j,k), 1 <= k <= Nz-2, and, for variable
- * density, the gradient term (src/cu/pressure.cu:14-101, 103-327).
+/* p-independent part of f at node (i,j,k), 1 <= k <= Nz-2
+ * (src/cu/pressure.cu:14-101 for constant density, 103-327 otherwise).
  * [Q1] h_c per node with |T_gas - T_solid|. */
 __device__ double wf_f_fixed(const WfConst &c, const WfState &Ys, const WfState &Yn, int i, int j, int k) {
     const int Nx = c.Nx, Ny = c.Ny;
@@ -92,8 +95,9 @@ __device__ double wf_f_fixed(const WfConst &c, const WfState &Ys, const WfState 
     return rho * (ux + vy + wz - div_U_temp) / c.dt;
 }
 
-/* (grad rho . grad p) / rho at node (i,j,k), 1 <= k <= Nz-2, rho from T^n. */
-__device__ double wf_f_pgrad(const WfConst &c, const WfState &Yn, const double *p, int i, int j, int k) {
+/* (grad rho . grad p) / rho at node (i,j,k), 1 <= k <= Nz-2, with rho = rho^n
+ * the density of y_n (src/cu/pressure.cu:163-197, 269-282, 319). */
+__device__ double wf_f_pgrad(const WfConst &c, const double *rho_n, const double *p, int i, int j, int k) {
     const int Nx = c.Nx, Ny = c.Ny;
     const long P = c.plane;
     const double *z = c.z;
@@ -102,32 +106,75 @@ __device__ double wf_f_pgrad(const WfConst &c, const WfState &Yn, const double *
     long q = wf_q(c, i, j, k);
     long qxm = wf_q(c, im1, j, k), qxp = wf_q(c, ip1, j, k), qym = wf_q(c, i, jm1, k), qyp = wf_q(c, i, jp1, k);
     double dz_km1 = z[k] - z[k - 1], dz_k = z[k + 1] - z[k];
-    double ri = c.T_inf * c.rho_inf;
-    double rho = ri / Yn.T[q];
-    double rho_ip = ri / Yn.T[qxp], rho_im = ri / Yn.T[qxm];
-    double rho_jp = ri / Yn.T[qyp], rho_jm = ri / Yn.T[qym];
-    double rho_kp = ri / Yn.T[q + P], rho_km = ri / Yn.T[q - P];
-    double rhox = (rho_ip - rho_im) / (2 * c.dx), rhoy = (rho_jp - rho_jm) / (2 * c.dy);
+    double rho = rho_n[q];
+    double rhox = (rho_n[qxp] - rho_n[qxm]) / (2 * c.dx), rhoy = (rho_n[qyp] - rho_n[qym]) / (2 * c.dy);
     double px = (p[qxp] - p[qxm]) / (2 * c.dx), py = (p[qyp] - p[qym]) / (2 * c.dy);
-    double rhoz = (rho_kp - rho_km) / (dz_k + dz_km1);
+    double rhoz = (rho_n[q + P] - rho_n[q - P]) / (dz_k + dz_km1);
     double pz = (p[q + P] - p[q - P]) / (dz_k + dz_km1);
     return (rhox * px + rhoy * py + rhoz * pz) / rho;
 }
 
+/* rho^n = T_inf rho_inf / T^n on every node, once per step. */
+__global__ void k_density(WfConst c, const double *T, double *rho) {
+    long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < c.N) rho[q] = c.T_inf * c.rho_inf / T[q];
+}
+
+__global__ void k_iter_reset(WfIterCtl *ctl) {
+    ctl->norm_bits = 0ull;
+    ctl->done = 0;
+    ctl->iters = 0;
+    ctl->solves = 0;
+    ctl->err = 0.0;
+}
+
+/* After solve m: err = max|p_new - p_old|, stop once err <= tol or after
+ * max_iter solves (src/cu/pressure.cu:564-574); iters is the count the
+ * reference reports (the index of the converging solve, else max_iter). */
+__global__ void k_iter_check(WfIterCtl *ctl, double tol, int m, int max_iter) {
+    if (ctl->done) return;
+    double err = __longlong_as_double((long long)ctl->norm_bits);
+    ctl->err = err;
+    ctl->solves = m + 1;
+    ctl->norm_bits = 0ull;
+    if (err <= tol) {
+        ctl->done = 1;
+        ctl->iters = m;
+    } else {
+        ctl->iters = m + 1;
+        if (m + 1 >= max_iter) ctl->done = 1;
+    }
+}
+
+/* Wavefront max of a non-negative double, folded into *bits (IEEE order of
+ * non-negative doubles equals the unsigned order of their bit patterns). */
+__device__ void wf_atomic_max_nonneg(unsigned long long *bits, double v) {
+    unsigned long long b = __double_as_longlong(v);
+    for (int off = 32; off > 0; off >>= 1) {
+        unsigned long long o = __shfl_xor(b, off, 64);
+        b = (o > b) ? o : b;
+    }
+    if ((threadIdx.x & 63) == 0) atomicMax(bits, b);
+}
+
 /* P1: f on slab k (1..nz-1) for sp.fx_pairs row pairs, forward x half-spectrum.
  * source 0: Poisson RHS (first_iter: also stores its p-independent part in sp.A);
- * source 1: the top pressure plane into sp.Ptop. */
+ * source 1: the top pressure plane into sp.Ptop.  NX > 0 selects the
+ * compile-time plan of that length. */
+template <int NX>
 __global__ __launch_bounds__(256) void k_rhs_fx(WfConst c, WfSpectral sp, WfState Ys, WfState Yn, const double *p,
                                                 int first_iter, int source) {
+    if (source == 0 && sp.ctl->done) return;
     extern __shared__ wfc smem[];
     __shared__ int rad[WF_MAX_RADICES];
-    const int nx = sp.nx, ny = sp.ny, hx = sp.hx;
-    wfc *tw = smem, *bufa = smem + nx, *bufb = bufa + sp.fx_pairs * nx;
+    const int nx = (NX > 0) ? NX : sp.nx;
+    const int ny = sp.ny, hx = sp.hx, pairs = sp.fx_pairs;
+    wfc *tw = smem, *bufa = smem + nx, *bufb = bufa + pairs * nx;
     int k = (source == 1) ? c.Nz - 1 : 1 + blockIdx.y;
-    int jpair0 = blockIdx.x * sp.fx_pairs;
+    int jpair0 = blockIdx.x * pairs;
     for (int t = threadIdx.x; t < nx; t += blockDim.x) tw[t] = *(const wfc *)&sp.twx[t];
-    if (threadIdx.x < WF_MAX_RADICES) rad[threadIdx.x] = sp.radx[threadIdx.x];
-    for (int t = threadIdx.x; t < sp.fx_pairs * nx; t += blockDim.x) {
+    if (NX == 0 && threadIdx.x < WF_MAX_RADICES) rad[threadIdx.x] = sp.radx[threadIdx.x];
+    for (int t = threadIdx.x; t < pairs * nx; t += blockDim.x) {
         int s = t / nx, i = t - s * nx;
         int ja = 2 * (jpair0 + s), jb = ja + 1;
         double fa = 0.0, fb = 0.0;
@@ -143,7 +190,7 @@ __global__ __launch_bounds__(256) void k_rhs_fx(WfConst c, WfSpectral sp, WfStat
                 } else {
                     fa = sp.A[qa];
                 }
-                if (c.variable_density) fa = fa + wf_f_pgrad(c, Yn, p, i, ja, k);
+                if (c.variable_density) fa = fa + wf_f_pgrad(c, sp.rho, p, i, ja, k);
                 if (jb < ny) {
                     long qb = wf_q(c, i, jb, k);
                     if (first_iter || c.variable_density == 0) {
@@ -152,7 +199,7 @@ __global__ __launch_bounds__(256) void k_rhs_fx(WfConst c, WfSpectral sp, WfStat
                     } else {
                         fb = sp.A[qb];
                     }
-                    if (c.variable_density) fb = fb + wf_f_pgrad(c, Yn, p, i, jb, k);
+                    if (c.variable_density) fb = fb + wf_f_pgrad(c, sp.rho, p, i, jb, k);
                 }
             }
         }
@@ -160,9 +207,13 @@ __global__ __launch_bounds__(256) void k_rhs_fx(WfConst c, WfSpectral sp, WfStat
         bufa[s * nx + i].im = fb;
     }
     __syncthreads();
-    wfc *res = wf_lds_fft(bufa, bufb, nx, sp.fx_pairs, rad, tw);
+    wfc *res;
+    if constexpr (NX > 0)
+        res = wf_fft_ct<NX>(bufa, bufb, pairs, tw);
+    else
+        res = wf_lds_fft(bufa, bufb, nx, pairs, rad, tw);
     double2 *dst = (source == 1) ? sp.Ptop : sp.F + (long)k * ny * hx;
-    for (int t = threadIdx.x; t < sp.fx_pairs * hx; t += blockDim.x) {
+    for (int t = threadIdx.x; t < pairs * hx; t += blockDim.x) {
         int s = t / hx, r = t - s * hx;
         int ja = 2 * (jpair0 + s), jb = ja + 1;
         if (ja >= ny) continue;
@@ -175,21 +226,24 @@ __global__ __launch_bounds__(256) void k_rhs_fx(WfConst c, WfSpectral sp, WfStat
 }
 
 /* P2/P4: y-transform of sp.fy_cols kx columns over all ky, one slab. */
-__global__ __launch_bounds__(256) void k_fy(WfSpectral sp, double2 *buf, int slab0, int sign) {
+template <int NY>
+__global__ __launch_bounds__(256) void k_fy(WfSpectral sp, double2 *buf, int slab0, int sign, int guarded) {
+    if (guarded && sp.ctl->done) return;
     extern __shared__ wfc smem[];
     __shared__ int rad[WF_MAX_RADICES];
-    const int ny = sp.ny, hx = sp.hx;
-    wfc *tw = smem, *bufa = smem + ny, *bufb = bufa + sp.fy_cols * ny;
-    int r0 = blockIdx.x * sp.fy_cols;
+    const int ny = (NY > 0) ? NY : sp.ny;
+    const int hx = sp.hx, cols = sp.fy_cols;
+    wfc *tw = smem, *bufa = smem + ny, *bufb = bufa + cols * ny;
+    int r0 = blockIdx.x * cols;
     long slab = (long)(slab0 + blockIdx.y) * ny * hx;
     for (int t = threadIdx.x; t < ny; t += blockDim.x) {
         double2 w = sp.twy[t];
         tw[t].re = w.x;
         tw[t].im = (sign > 0) ? -w.y : w.y;
     }
-    if (threadIdx.x < WF_MAX_RADICES) rad[threadIdx.x] = sp.rady[threadIdx.x];
-    for (int t = threadIdx.x; t < sp.fy_cols * ny; t += blockDim.x) {
-        int cc = t % sp.fy_cols, jj = t / sp.fy_cols;
+    if (NY == 0 && threadIdx.x < WF_MAX_RADICES) rad[threadIdx.x] = sp.rady[threadIdx.x];
+    for (int t = threadIdx.x; t < cols * ny; t += blockDim.x) {
+        int cc = t % cols, jj = t / cols;
         int r = r0 + cc;
         wfc v = {0.0, 0.0};
         if (r < hx) {
@@ -200,9 +254,13 @@ __global__ __launch_bounds__(256) void k_fy(WfSpectral sp, double2 *buf, int sla
         bufa[cc * ny + jj] = v;
     }
     __syncthreads();
-    wfc *res = wf_lds_fft(bufa, bufb, ny, sp.fy_cols, rad, tw);
-    for (int t = threadIdx.x; t < sp.fy_cols * ny; t += blockDim.x) {
-        int cc = t % sp.fy_cols, jj = t / sp.fy_cols;
+    wfc *res;
+    if constexpr (NY > 0)
+        res = wf_fft_ct<NY>(bufa, bufb, cols, tw);
+    else
+        res = wf_lds_fft(bufa, bufb, ny, cols, rad, tw);
+    for (int t = threadIdx.x; t < cols * ny; t += blockDim.x) {
+        int cc = t % cols, jj = t / cols;
         int r = r0 + cc;
         if (r < hx) buf[slab + (long)jj * hx + r] = make_double2(res[cc * ny + jj].re, res[cc * ny + jj].im);
     }
@@ -234,6 +292,7 @@ __global__ void k_pivots(WfSpectral sp, double *U) {
 /* P3: forward elimination and back substitution per (kx, ky), in place in sp.F
  * (src/cu/pressure.cu:418-443 with the right-hand side of 473-504). */
 __global__ void k_thomas(WfSpectral sp) {
+    if (sp.ctl->done) return;
     int r = blockIdx.x * blockDim.x + threadIdx.x;
     int s = blockIdx.y;
     if (r >= sp.hx) return;
@@ -273,35 +332,51 @@ __global__ void k_thomas(WfSpectral sp) {
     }
 }
 
-/* Wavefront max of a non-negative double, folded into *bits (IEEE order of
- * non-negative doubles equals the unsigned order of their bit patterns). */
-__device__ void wf_atomic_max_nonneg(unsigned long long *bits, double v) {
-    unsigned long long b = __double_as_longlong(v);
-    for (int off = 32; off > 0; off >>= 1) {
-        unsigned long long o = __shfl_xor(b, off, 64);
-        b = (o > b) ? o : b;
-    }
-    if ((threadIdx.x & 63) == 0) atomicMax(bits, b);
-}
-
 /* P5: inverse x-transform of sp.fx_pairs row pairs on slab k (0..nz-1):
  * p = Re(.)/(nx ny) on unique nodes, duplicate planes i = Nx-1 and j = Ny-1 set
- * from their images [Q6], and the running max |p_new - p_old|. */
+ * from their images [Q6], and the running max |p_new - p_old| into the
+ * iteration control.  Workgroup row blockIdx.y = nz refreshes the periodic
+ * copies on the top plane, which the solve never writes on unique nodes. */
+template <int NX>
 __global__ __launch_bounds__(256) void k_inv_fx(WfConst c, WfSpectral sp, double *p) {
+    if (sp.ctl->done) return;
     extern __shared__ wfc smem[];
     __shared__ int rad[WF_MAX_RADICES];
-    const int nx = sp.nx, ny = sp.ny, hx = sp.hx, Nx = c.Nx;
-    wfc *tw = smem, *bufa = smem + nx, *bufb = bufa + sp.fx_pairs * nx;
+    const int nx = (NX > 0) ? NX : sp.nx;
+    const int ny = sp.ny, hx = sp.hx, Nx = c.Nx, pairs = sp.fx_pairs;
     int k = blockIdx.y;
-    int jpair0 = blockIdx.x * sp.fx_pairs;
+    double dmax = 0.0;
+    if (k == sp.nz) {
+        if (blockIdx.x == 0) {
+            int kt = c.Nz - 1;
+            for (int t = threadIdx.x; t < c.Ny + c.Nx - 1; t += blockDim.x) {
+                long qd, qs;
+                if (t < c.Ny) { /* i = Nx-1 from i = 0 */
+                    qd = wf_q(c, c.Nx - 1, t, kt);
+                    qs = wf_q(c, 0, (t == c.Ny - 1) ? 0 : t, kt);
+                } else { /* j = Ny-1 from j = 0 */
+                    qd = wf_q(c, t - c.Ny, c.Ny - 1, kt);
+                    qs = wf_q(c, t - c.Ny, 0, kt);
+                }
+                double v = p[qs];
+                double d = fabs(v - p[qd]);
+                dmax = (d > dmax) ? d : dmax;
+                p[qd] = v;
+            }
+        }
+        wf_atomic_max_nonneg(&sp.ctl->norm_bits, dmax);
+        return;
+    }
+    wfc *tw = smem, *bufa = smem + nx, *bufb = bufa + pairs * nx;
+    int jpair0 = blockIdx.x * pairs;
     const double2 *src = sp.F + (long)k * ny * hx;
     for (int t = threadIdx.x; t < nx; t += blockDim.x) {
         double2 w = sp.twx[t];
         tw[t].re = w.x;
         tw[t].im = -w.y;
     }
-    if (threadIdx.x < WF_MAX_RADICES) rad[threadIdx.x] = sp.radx[threadIdx.x];
-    for (int t = threadIdx.x; t < sp.fx_pairs * nx; t += blockDim.x) {
+    if (NX == 0 && threadIdx.x < WF_MAX_RADICES) rad[threadIdx.x] = sp.radx[threadIdx.x];
+    for (int t = threadIdx.x; t < pairs * nx; t += blockDim.x) {
         int s = t / nx, r = t - s * nx;
         int ja = 2 * (jpair0 + s), jb = ja + 1;
         wfc A = {0.0, 0.0}, B = {0.0, 0.0};
@@ -320,10 +395,13 @@ __global__ __launch_bounds__(256) void k_inv_fx(WfConst c, WfSpectral sp, double
         bufa[s * nx + r] = wf_pack_pair(A, B, conj);
     }
     __syncthreads();
-    wfc *res = wf_lds_fft(bufa, bufb, nx, sp.fx_pairs, rad, tw);
+    wfc *res;
+    if constexpr (NX > 0)
+        res = wf_fft_ct<NX>(bufa, bufb, pairs, tw);
+    else
+        res = wf_lds_fft(bufa, bufb, nx, pairs, rad, tw);
     double scale = (double)nx * (double)ny;
-    double dmax = 0.0;
-    for (int t = threadIdx.x; t < sp.fx_pairs * Nx; t += blockDim.x) {
+    for (int t = threadIdx.x; t < pairs * Nx; t += blockDim.x) {
         int s = t / Nx, i = t - s * Nx;
         int ja = 2 * (jpair0 + s), jb = ja + 1;
         if (ja >= ny) continue;
@@ -347,39 +425,43 @@ __global__ __launch_bounds__(256) void k_inv_fx(WfConst c, WfSpectral sp, double
             p[qb] = pb;
         }
     }
-    wf_atomic_max_nonneg(sp.norm_bits, dmax);
-}
-
-/* Duplicate nodes of the top plane (k = Nz-1), which the solve does not write
- * on the unique nodes but whose periodic copies it refreshes. */
-__global__ void k_top_dup(WfConst c, WfSpectral sp, double *p) {
-    int t = blockIdx.x * blockDim.x + threadIdx.x;
-    int k = c.Nz - 1;
-    double dmax = 0.0;
-    if (t < c.Ny) { /* i = Nx-1 from i = 0 */
-        int j = (t == c.Ny - 1) ? 0 : t;
-        long qd = wf_q(c, c.Nx - 1, t, k);
-        double v = p[wf_q(c, 0, j, k)];
-        dmax = fabs(v - p[qd]);
-        p[qd] = v;
-    } else if (t < c.Ny + c.Nx - 1) { /* j = Ny-1 from j = 0, i < Nx-1 */
-        int i = t - c.Ny;
-        long qd = wf_q(c, i, c.Ny - 1, k);
-        double v = p[wf_q(c, i, 0, k)];
-        dmax = fabs(v - p[qd]);
-        p[qd] = v;
-    }
-    wf_atomic_max_nonneg(sp.norm_bits, dmax);
+    wf_atomic_max_nonneg(&sp.ctl->norm_bits, dmax);
 }
 
 /* ------------------------------------------------------------- launchers ---- */
+
+#define WF_DISPATCH_N(n, KERNEL, grd, blk, lds, s, ...)                                        \
+    do {                                                                                      \
+        switch (n) {                                                                          \
+            case 63: hipLaunchKernelGGL(KERNEL<63>, grd, blk, lds, s, __VA_ARGS__); break;    \
+            case 127: hipLaunchKernelGGL(KERNEL<127>, grd, blk, lds, s, __VA_ARGS__); break;  \
+            case 255: hipLaunchKernelGGL(KERNEL<255>, grd, blk, lds, s, __VA_ARGS__); break;  \
+            case 511: hipLaunchKernelGGL(KERNEL<511>, grd, blk, lds, s, __VA_ARGS__); break;  \
+            default: hipLaunchKernelGGL(KERNEL<0>, grd, blk, lds, s, __VA_ARGS__); break;     \
+        }                                                                                     \
+    } while (0)
+
+hipError_t wf_launch_density(const WfConst &c, const double *T, double *rho, hipStream_t s) {
+    hipLaunchKernelGGL(k_density, dim3((unsigned)((c.N + 255) / 256)), dim3(256), 0, s, c, T, rho);
+    return hipGetLastError();
+}
+
+hipError_t wf_launch_iter_reset(WfIterCtl *ctl, hipStream_t s) {
+    hipLaunchKernelGGL(k_iter_reset, dim3(1), dim3(1), 0, s, ctl);
+    return hipGetLastError();
+}
+
+hipError_t wf_launch_iter_check(WfIterCtl *ctl, double tol, int m, int max_iter, hipStream_t s) {
+    hipLaunchKernelGGL(k_iter_check, dim3(1), dim3(1), 0, s, ctl, tol, m, max_iter);
+    return hipGetLastError();
+}
 
 hipError_t wf_launch_pressure_rhs_fx(const WfConst &c, const WfSpectral &sp, const WfState &Ys, const WfState &Yn,
                                      const double *p, int first_iter, hipStream_t s) {
     int npairs = (sp.ny + 1) / 2;
     dim3 grd((npairs + sp.fx_pairs - 1) / sp.fx_pairs, sp.nz - 1, 1);
     size_t lds = sizeof(wfc) * (size_t)sp.nx * (1 + 2 * sp.fx_pairs);
-    hipLaunchKernelGGL(k_rhs_fx, grd, dim3(256), lds, s, c, sp, Ys, Yn, p, first_iter, 0);
+    WF_DISPATCH_N(sp.nx, k_rhs_fx, grd, dim3(256), lds, s, c, sp, Ys, Yn, p, first_iter, 0);
     return hipGetLastError();
 }
 
@@ -388,15 +470,16 @@ hipError_t wf_launch_ptop_fx(const WfConst &c, const WfSpectral &sp, const doubl
     dim3 grd((npairs + sp.fx_pairs - 1) / sp.fx_pairs, 1, 1);
     WfState none = {0, 0, 0, 0, 0};
     size_t lds = sizeof(wfc) * (size_t)sp.nx * (1 + 2 * sp.fx_pairs);
-    hipLaunchKernelGGL(k_rhs_fx, grd, dim3(256), lds, s, c, sp, none, none, p, 0, 1);
+    WF_DISPATCH_N(sp.nx, k_rhs_fx, grd, dim3(256), lds, s, c, sp, none, none, p, 0, 1);
     return hipGetLastError();
 }
 
-hipError_t wf_launch_fy(const WfSpectral &sp, double2 *buf, int slab0, int nslabs, int sign, hipStream_t s) {
+hipError_t wf_launch_fy(const WfSpectral &sp, double2 *buf, int slab0, int nslabs, int sign, int guarded,
+                        hipStream_t s) {
     if (nslabs <= 0) return hipSuccess;
     dim3 grd((sp.hx + sp.fy_cols - 1) / sp.fy_cols, nslabs, 1);
     size_t lds = sizeof(wfc) * (size_t)sp.ny * (1 + 2 * sp.fy_cols);
-    hipLaunchKernelGGL(k_fy, grd, dim3(256), lds, s, sp, buf, slab0, sign);
+    WF_DISPATCH_N(sp.ny, k_fy, grd, dim3(256), lds, s, sp, buf, slab0, sign, guarded);
     return hipGetLastError();
 }
 
@@ -414,14 +497,8 @@ hipError_t wf_launch_thomas(const WfSpectral &sp, hipStream_t s) {
 
 hipError_t wf_launch_inverse_fx(const WfConst &c, const WfSpectral &sp, double *p, hipStream_t s) {
     int npairs = (sp.ny + 1) / 2;
-    dim3 grd((npairs + sp.fx_pairs - 1) / sp.fx_pairs, sp.nz, 1);
+    dim3 grd((npairs + sp.fx_pairs - 1) / sp.fx_pairs, sp.nz + 1, 1);
     size_t lds = sizeof(wfc) * (size_t)sp.nx * (1 + 2 * sp.fx_pairs);
-    hipLaunchKernelGGL(k_inv_fx, grd, dim3(256), lds, s, c, sp, p);
-    return hipGetLastError();
-}
-
-hipError_t wf_launch_top_duplicates(const WfConst &c, const WfSpectral &sp, double *p, hipStream_t s) {
-    int n = c.Ny + c.Nx - 1;
-    hipLaunchKernelGGL(k_top_dup, dim3((n + 63) / 64), dim3(64), 0, s, c, sp, p);
+    WF_DISPATCH_N(sp.nx, k_inv_fx, grd, dim3(256), lds, s, c, sp, p);
     return hipGetLastError();
 }

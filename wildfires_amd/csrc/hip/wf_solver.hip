@@ -60,9 +60,9 @@ struct wf_solver {
     double *z, *cd2, *topo, *kx2, *ky2, *ta, *tc, *beta, *x, *y;
     int *cut, *nzy, *radx, *rady;
     double2 *twx, *twy;
-    double *scratch, *Tsrc;
+    double *scratch, *Tsrc, *rho;
     int have_Tsrc;
-    unsigned long long *norm;
+    WfIterCtl *ctl;
     int method; /* 0 Euler, 1 RK2, 2 RK4 */
     long press_solves, steps;
     double last_err;
@@ -217,7 +217,9 @@ static int wf_create_impl(wf_solver *s, const Parameters *P) {
     WF_HIP(hipMalloc((void **)&s->A, sizeof(double) * s->N));
     WF_HIP(hipMalloc((void **)&s->scratch, sizeof(double) * s->S));
     WF_HIP(hipMalloc((void **)&s->Tsrc, sizeof(double) * s->N));
-    WF_HIP(hipMalloc((void **)&s->norm, sizeof(unsigned long long)));
+    WF_HIP(hipMalloc((void **)&s->rho, sizeof(double) * s->N));
+    WF_HIP(hipMalloc((void **)&s->ctl, sizeof(WfIterCtl)));
+    WF_HIP(hipMemset(s->ctl, 0, sizeof(WfIterCtl)));
 
     /* grid tables evaluated on the host exactly as the reference evaluates them */
     const double *z = P->z;
@@ -364,7 +366,8 @@ static int wf_create_impl(wf_solver *s, const Parameters *P) {
     sp.twy = s->twy;
     sp.radx = s->radx;
     sp.rady = s->rady;
-    sp.norm_bits = s->norm;
+    sp.rho = s->rho;
+    sp.ctl = s->ctl;
     WF_HIP(wf_launch_pivots(sp, s->U, s->stream));
     WF_HIP(hipStreamSynchronize(s->stream));
     for (int q = 0; q < 2 * WF_MAX_EVENTS; q++) WF_HIP(hipEventCreate(&s->ev[q]));
@@ -394,7 +397,7 @@ extern "C" int wf_solver_destroy(wf_solver *s) {
     for (int q = 0; q < 4; q++) (void)hipFree(s->state[q]);
     void *bufs[] = {s->derived, s->p, s->A, s->U, s->F, s->Ptop, s->z, s->cd2, s->topo, s->kx2, s->ky2, s->ta,
                     s->tc, s->beta, s->x, s->y, s->cut, s->nzy, s->radx, s->rady, s->twx, s->twy, s->scratch,
-                    s->Tsrc, s->norm};
+                    s->Tsrc, s->rho, s->ctl};
     for (size_t q = 0; q < sizeof(bufs) / sizeof(bufs[0]); q++)
         if (bufs[q]) (void)hipFree(bufs[q]);
     for (int q = 0; q < 2 * WF_MAX_EVENTS; q++)
@@ -445,7 +448,7 @@ static int wf_get_field(wf_solver *s, const double *src, double *f_host) {
  * writes the unique nodes of k = Nz-1) */
 static int wf_top_spectrum(wf_solver *s) {
     WF_HIP(wf_launch_ptop_fx(s->c, s->sp, s->p, s->stream));
-    WF_HIP(wf_launch_fy(s->sp, s->Ptop, 0, 1, -1, s->stream));
+    WF_HIP(wf_launch_fy(s->sp, s->Ptop, 0, 1, -1, 0, s->stream));
     return WF_OK;
 }
 
@@ -505,31 +508,47 @@ static int wf_integrate(wf_solver *s) {
     return WF_OK;
 }
 
-/* Pressure solve of src/cu/pressure.cu:506-606 for y* = Ys, y_n = Yn. */
+/* Pressure solve of src/cu/pressure.cu:506-606 for y* = Ys, y_n = Yn.  The
+ * solves are queued without host synchronisation; the convergence test runs
+ * on the device after each solve and turns the remaining kernels into no-ops.
+ * The host looks at the flag every WF_POLL solves to stop queueing. */
+#define WF_POLL 8
 static int wf_pressure_solve(wf_solver *s, double *Ys_buf, double *Yn_buf) {
     const Parameters &P = s->P;
     WfState ys = wf_view(s, Ys_buf), yn = wf_view(s, Yn_buf);
-    int iters = (P.variable_density == 0) ? 1 : P.pressure_solver_iter;
-    double err = 0.0;
-    int m;
-    for (m = 0; m < iters; m++) {
-        WF_HIP(hipMemsetAsync(s->norm, 0, sizeof(unsigned long long), s->stream));
-        WF_LAUNCH(s, WF_K_PRHS, wf_launch_pressure_rhs_fx(s->c, s->sp, ys, yn, s->p, m == 0, s->stream));
-        WF_LAUNCH(s, WF_K_PFY, wf_launch_fy(s->sp, s->F, 1, s->sp.nz - 1, -1, s->stream));
-        WF_LAUNCH(s, WF_K_THOMAS, wf_launch_thomas(s->sp, s->stream));
-        WF_LAUNCH(s, WF_K_PFY, wf_launch_fy(s->sp, s->F, 0, s->sp.nz, +1, s->stream));
-        WF_LAUNCH(s, WF_K_PINV, wf_launch_inverse_fx(s->c, s->sp, s->p, s->stream));
-        WF_LAUNCH(s, WF_K_OTHER, wf_launch_top_duplicates(s->c, s->sp, s->p, s->stream));
-        s->press_solves++;
-        if (P.variable_density == 0) break;
-        unsigned long long bits = 0;
-        WF_HIP(hipMemcpyAsync(&bits, s->norm, sizeof(bits), hipMemcpyDeviceToHost, s->stream));
-        WF_HIP(hipStreamSynchronize(s->stream));
-        memcpy(&err, &bits, sizeof(err));
-        if (err <= P.pressure_solver_tol) break;
+    int max_iter = (P.variable_density == 0) ? 1 : P.pressure_solver_iter;
+    WF_LAUNCH(s, WF_K_OTHER, wf_launch_iter_reset(s->ctl, s->stream));
+    if (max_iter <= 0) {
+        s->last_err = 0.0;
+        s->last_iters = 0;
+        return WF_OK;
     }
-    s->last_err = err;
-    s->last_iters = m;
+    if (P.variable_density) WF_LAUNCH(s, WF_K_OTHER, wf_launch_density(s->c, yn.T, s->rho, s->stream));
+    for (int m = 0; m < max_iter; m++) {
+        WF_LAUNCH(s, WF_K_PRHS, wf_launch_pressure_rhs_fx(s->c, s->sp, ys, yn, s->p, m == 0, s->stream));
+        WF_LAUNCH(s, WF_K_PFY, wf_launch_fy(s->sp, s->F, 1, s->sp.nz - 1, -1, 1, s->stream));
+        WF_LAUNCH(s, WF_K_THOMAS, wf_launch_thomas(s->sp, s->stream));
+        WF_LAUNCH(s, WF_K_PFY, wf_launch_fy(s->sp, s->F, 0, s->sp.nz, +1, 1, s->stream));
+        WF_LAUNCH(s, WF_K_PINV, wf_launch_inverse_fx(s->c, s->sp, s->p, s->stream));
+        WF_LAUNCH(s, WF_K_OTHER, wf_launch_iter_check(s->ctl, P.pressure_solver_tol, m, max_iter, s->stream));
+        if ((m + 1) % WF_POLL == 0 && m + 1 < max_iter) {
+            int done = 0;
+            WF_HIP(hipMemcpyAsync(&done, &s->ctl->done, sizeof(int), hipMemcpyDeviceToHost, s->stream));
+            WF_HIP(hipStreamSynchronize(s->stream));
+            if (done) break;
+        }
+    }
+    WfIterCtl ctl;
+    WF_HIP(hipMemcpyAsync(&ctl, s->ctl, sizeof(ctl), hipMemcpyDeviceToHost, s->stream));
+    WF_HIP(hipStreamSynchronize(s->stream));
+    s->press_solves += ctl.solves;
+    if (P.variable_density == 0) {
+        s->last_err = 0.0;
+        s->last_iters = 0;
+    } else {
+        s->last_err = ctl.err;
+        s->last_iters = ctl.iters;
+    }
     return WF_OK;
 }
 

@@ -340,10 +340,17 @@ Parameters read_parameters_file(const char *file_path) {
     }
     generate_current_datetime_string(parameters.sim_id, 32);
     const char *root = getenv("WILDFIRES_OUTPUT_ROOT");
+    int need;
     if (root != NULL && root[0] != '\0')
-        snprintf(parameters.save_path, sizeof(parameters.save_path), "%s/%s/", root, parameters.sim_id);
+        need = snprintf(parameters.save_path, sizeof(parameters.save_path), "%s/%s/", root, parameters.sim_id);
     else
-        snprintf(parameters.save_path, sizeof(parameters.save_path), "data/output/%s/", parameters.sim_id);
+        need = snprintf(parameters.save_path, sizeof(parameters.save_path), "data/output/%s/", parameters.sim_id);
+    if (need >= (int)sizeof(parameters.save_path)) {
+        /* save_path is a 64-byte member of Parameters (include/c/structures.h:238) */
+        fprintf(stderr, "Output directory path too long (%d characters, at most %d)\n", need,
+                (int)sizeof(parameters.save_path) - 1);
+        exit(1);
+    }
     wf_make_dirs(parameters.save_path);
     snprintf(path, sizeof(path), "%sparameters.txt", parameters.save_path);
     parameters.log_files.parameters = fopen(path, "w");
