@@ -61,7 +61,8 @@ def kernel_bytes(params, stats):
     later = d * 3 * N + spec
     out["pressure_rhs_fx"] = (first + (it - 1) * later) / it
     out["pressure_fy"] = 2 * spec
-    out["thomas"] = 2 * spec + 2 * spec + 2 * 8.0 * hx * ny * nz   # d, y (w+r), x, pivots twice
+    # d in, x out, forward multipliers and reciprocal pivots (real) in
+    out["thomas"] = 2 * spec + 2 * 8.0 * hx * ny * nz
     out["pressure_inverse_fx"] = spec + 2 * d * N
     out["project"] = d * (S + N + N + S)
     return out
@@ -73,7 +74,7 @@ def run(args):
 
     wd = tempfile.mkdtemp(prefix="wf_bench_")
     path = os.path.join(wd, args.workload + ".txt")
-    total = args.warmup + args.steps + args.profile_steps
+    total = args.warmup + args.steps + max(args.profile_steps, 1)
     configs.write_parameter_file(path, args.workload, Nt=max(total, 2), NT=max(total, 2),
                                  t_max=0.005 * max(total, 2) if args.workload == "hill256" else 0.01 * max(total, 2))
     params = api.read_parameters(path, output_root=os.path.join(wd, "out"))
@@ -106,7 +107,7 @@ def run(args):
 
     # per-kernel timing (HIP events on the solver stream) over profile_steps more steps
     solver.profile(True)
-    solver.step(n, args.profile_steps)
+    solver.step(n, max(args.profile_steps, 1))
     times = solver.kernel_times()
     solver.profile(False)
     kbytes = kernel_bytes(params, {"solves_per_step": solves})
@@ -177,6 +178,66 @@ def cpu_baseline(params, y0, p0):
                       % (solves, params.p.Nx, params.p.Ny, params.p.Nz)}
 
 
+def copy_calibration(nbytes, repeats=3):
+    """Device-to-device copies of a known size through the HIP runtime's copy
+    kernel: under rocprofv3 --pmc its FETCH_SIZE / WRITE_SIZE per dispatch
+    against nbytes give the counters' scale factor."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    if hip.hipMalloc(ctypes.byref(a), ctypes.c_size_t(nbytes)) or hip.hipMalloc(ctypes.byref(b), ctypes.c_size_t(nbytes)):
+        raise RuntimeError("hipMalloc of %d bytes failed" % nbytes)
+    for _ in range(repeats):
+        if hip.hipMemcpy(b, a, ctypes.c_size_t(nbytes), 3):  # hipMemcpyDeviceToDevice
+            raise RuntimeError("hipMemcpy failed")
+    hip.hipDeviceSynchronize()
+    hip.hipFree(a)
+    hip.hipFree(b)
+    log("copy calibration: %d device-to-device copies of %d bytes" % (repeats, nbytes))
+
+
+def _short_name(name):
+    name = name.strip().strip('"')
+    if name.startswith("void "):
+        name = name[5:]
+    return name.split("(")[0]
+
+
+def rocprof_summary(directory):
+    """Markdown summary of the CSV output of rocprofv3 under `directory`:
+    kernel statistics (--stats) and per-dispatch counters (--pmc)."""
+    import csv
+    import glob
+    lines = []
+    stats = sorted(glob.glob(os.path.join(directory, "**", "*kernel_stats.csv"), recursive=True))
+    for path in stats:
+        with open(path) as f:
+            rows = list(csv.DictReader(f))
+        lines.append("| kernel | calls | mean us | total ms | % |")
+        lines.append("|---|---:|---:|---:|---:|")
+        for r in rows:
+            lines.append("| %s | %s | %.2f | %.3f | %.1f |" % (
+                _short_name(r["Name"]), r["Calls"], float(r["AverageNs"]) / 1e3,
+                float(r["TotalDurationNs"]) / 1e6, float(r["Percentage"])))
+        lines.append("")
+    pmc = sorted(glob.glob(os.path.join(directory, "**", "*counter_collection.csv"), recursive=True))
+    for path in pmc:
+        with open(path) as f:
+            rows = list(csv.DictReader(f))
+        acc = {}
+        for r in rows:
+            key = (_short_name(r["Kernel_Name"]), r["Counter_Name"])
+            s = acc.setdefault(key, [0.0, 0, set()])
+            s[0] += float(r["Counter_Value"])
+            s[2].add(r.get("Dispatch_Id", r.get("Correlation_Id")))
+        lines.append("| kernel | counter | dispatches | mean per dispatch |")
+        lines.append("|---|---|---:|---:|")
+        for (k, cname), (tot, _, disp) in sorted(acc.items()):
+            lines.append("| %s | %s | %d | %.1f |" % (k, cname, len(disp), tot / max(len(disp), 1)))
+        lines.append("")
+    return "\n".join(lines)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -185,11 +246,21 @@ def main():
     ap.add_argument("--profile-steps", type=int, default=2)
     ap.add_argument("--workload", default="hill256", choices=["hill256", "flat64"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
+                    help="only run device-to-device copies of BYTES (counter scale factor under rocprofv3 --pmc)")
+    ap.add_argument("--rocprof-summary", metavar="DIR",
+                    help="print a markdown summary of rocprofv3 CSV output under DIR and exit")
     args = ap.parse_args()
+    if args.rocprof_summary:
+        print(rocprof_summary(args.rocprof_summary))
+        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (every box has one GPU)",
               file=sys.stderr)
         sys.exit(2)
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
+        return
     print(json.dumps(run(args)), flush=True)
 
 

@@ -12,13 +12,16 @@ and this library's `wildfires_amd` driver.
              terrain with Y ~ U[0.8, 1.0] (seeded), power-law wind u_r = 6 m/s,
              a line ignition T = 1000 K for 99 <= x <= 101 m over all y up to
              1 m above the terrain, RK4, dt = 0.005 s, 1000 steps.
-Burning workloads use A = 5e5 1/s, Y_f = 10 and T_max = 1500 K (FIRE_KEYS).
-With the parser defaults (A = 1e9) the Arrhenius heat release of a 1000 K
-fuel node grows by ~5e3 1/s, 27 times the RK4 stability limit at dt = 0.005 s:
-the stage temperatures reach 1e16 K and the state is NaN after one step.
-These values are the largest tested that stay finite for 1000 steps.
   small_*  : the same physics on grids the CPU oracle steps in well under a
-             second, used by the parity tests.
+             second (transform lengths 19 and 23: run-time radix plans).
+  mid_hill64, wide_flat256 : parity grids whose transform lengths (63 and 255)
+             take the compile-time plans of the benchmark grids; mid_hill64 also
+             splits its 39 z-levels over five tridiagonal chunks.
+Burning workloads (BURNING) use A = 5e5 1/s, Y_f = 10 and T_max = 1500 K
+(FIRE_KEYS).  With the parser defaults (A = 1e9) the Arrhenius heat release of
+a 1000 K fuel node grows by ~5e3 1/s, 27 times the RK4 stability limit at
+dt = 0.005 s: the stage temperatures reach 1e16 K and the state is NaN after
+one step.  These values are the largest tested that stay finite for 1000 steps.
 """
 import math
 import os
@@ -66,10 +69,19 @@ WORKLOADS = {
                        topo_shape="flat", u_r=4.0, Y_h=1.2, Y0_x_start=0.0, Y0_x_end=40.0,
                        Y0_y_start=0.0, Y0_y_end=48.0, Y0_xa=1.0, Y0_ya=1.0, T0_x_start=10.0, T0_x_end=14.0,
                        T0_y_start=16.0, T0_y_end=30.0, T0_z_start=0.0, T0_z_end=2.0),
+    "mid_hill64": dict(Nx=64, Ny=64, Nz=40, x_max=128.0, y_max=128.0, z_max=20.0, t_max=0.1, Nt=10, NT=10,
+                       topo_shape="hill", hill_center_x=64.0, hill_center_y=64.0, hill_length=20.0,
+                       hill_width=24.0, hill_height=5.0, u_r=3.0, Y_h=1.0, Y0_x_start=0.0,
+                       Y0_x_end=128.0, Y0_y_start=0.0, Y0_y_end=128.0, Y0_xa=1.0, Y0_ya=1.0, T0_x_start=20.0,
+                       T0_x_end=26.0, T0_y_start=20.0, T0_y_end=100.0, T0_z_start=0.0, T0_z_end=2.0),
+    "wide_flat256": dict(Nx=256, Ny=256, Nz=8, x_max=256.0, y_max=256.0, z_max=4.0, t_max=0.03, Nt=3, NT=3,
+                         topo_shape="flat", u_r=4.0, Y_h=1.2, Y0_x_start=0.0, Y0_x_end=256.0,
+                         Y0_y_start=0.0, Y0_y_end=256.0, Y0_xa=1.0, Y0_ya=1.0, T0_x_start=60.0, T0_x_end=70.0,
+                         T0_y_start=50.0, T0_y_end=200.0, T0_z_start=0.0, T0_z_end=2.0),
 }
 
 
-BURNING = ("hill256", "small_hill", "small_flat")
+BURNING = ("hill256", "small_hill", "small_flat", "mid_hill64", "wide_flat256")
 
 
 def parameter_text(name, **overrides):

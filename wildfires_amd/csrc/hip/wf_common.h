@@ -55,8 +55,10 @@ __device__ __forceinline__ long wf_q(const WfConst &c, int i, int j, int k) {
     return (long)i + (long)c.Nx * ((long)j + (long)c.Ny * (long)k);
 }
 
-__device__ __forceinline__ int wf_wrap_m(int i, int d, int n) { return (i - d + n - 1) % (n - 1); }
-__device__ __forceinline__ int wf_wrap_p(int i, int d, int n) { return (i + d) % (n - 1); }
+/* Periodic neighbours (i - d + n - 1) % (n - 1) and (i + d) % (n - 1) of the
+ * reference, for 0 <= i <= n-1 and d < n-1, without integer division. */
+__device__ __forceinline__ int wf_wrap_m(int i, int d, int n) { return (i - d >= 0) ? i - d : i - d + n - 1; }
+__device__ __forceinline__ int wf_wrap_p(int i, int d, int n) { return (i + d >= n - 1) ? i + d - (n - 1) : i + d; }
 
 /* Second-order one-sided first derivative from nodes k, k+1, k+2
  * (h0 = z[k+1]-z[k], h1 = z[k+2]-z[k+1]); src/cu/pde.cu:363-365. */

@@ -106,6 +106,7 @@ constexpr int wf_radix_of(int n, int idx) {
 }
 
 __device__ __forceinline__ wfc wf_cmul(wfc a, wfc b) {
+#pragma clang fp contract(fast)
     wfc r;
     r.re = a.re * b.re - a.im * b.im;
     r.im = a.re * b.im + a.im * b.re;
@@ -116,79 +117,182 @@ __device__ __forceinline__ wfc wf_cmul(wfc a, wfc b) {
  * sequences of length N (same mathematics as wf_stockham_output).  Radices up
  * to 8 run one butterfly per thread with its R values in registers; larger
  * prime radices run one output per thread to keep the workgroup busy. */
-template <int N, int R, int NS>
-__device__ __forceinline__ void wf_stage_ct(const wfc *in, wfc *out, int nseq, const wfc *tw) {
-    constexpr int M = N / R;
-    constexpr int SC = N / (NS * R);
-    constexpr int WR = N / R; /* W_R^m = tw[m * WR] */
-    if constexpr (R <= 8) {
-        for (int t = threadIdx.x; t < nseq * M; t += blockDim.x) {
-            int s = t / M, j = t - s * M;
-            int kq = j % NS;
-            const wfc *src = in + s * N + j;
-            wfc v[R];
+/* R-point DFT of x (twiddled inputs) into y, W_R^m = tw[m * WR]. */
+template <int R, int WR>
+__device__ __forceinline__ void wf_dft_generic(const wfc *x, wfc *y, const wfc *tw) {
+#pragma clang fp contract(fast)
 #pragma unroll
-            for (int r = 0; r < R; r++) v[r] = src[r * M];
+    for (int q = 0; q < R; q++) {
+        wfc acc = x[0];
 #pragma unroll
-            for (int r = 1; r < R; r++) v[r] = wf_cmul(v[r], tw[kq * r * SC]);
-            wfc *dst = out + s * N + (j / NS) * NS * R + kq;
-#pragma unroll
-            for (int q = 0; q < R; q++) {
-                wfc acc = v[0];
-#pragma unroll
-                for (int r = 1; r < R; r++) {
-                    wfc p = wf_cmul(v[r], tw[((r * q) % R) * WR]);
-                    acc.re += p.re;
-                    acc.im += p.im;
-                }
-                dst[q * NS] = acc;
-            }
+        for (int r = 1; r < R; r++) {
+            wfc p = wf_cmul(x[r], tw[((r * q) % R) * WR]);
+            acc.re += p.re;
+            acc.im += p.im;
         }
-    } else {
-        for (int t = threadIdx.x; t < nseq * N; t += blockDim.x) {
-            int s = t / N, o = t - s * N;
-            int kq = o % NS;
-            int rest = o / NS;
-            int q = rest % R;
-            int jb = (rest / R) * NS + kq;
-            int step = (kq + q * NS) * SC; /* < N */
-            const wfc *src = in + s * N + jb;
-            double ar = 0.0, ai = 0.0;
-            int e = 0;
-#pragma unroll 4
-            for (int r = 0; r < R; r++) {
-                wfc x = src[r * M];
-                wfc w = tw[e];
-                ar += w.re * x.re - w.im * x.im;
-                ai += w.re * x.im + w.im * x.re;
-                e += step;
-                if (e >= N) e -= N;
-            }
-            out[s * N + o].re = ar;
-            out[s * N + o].im = ai;
-        }
+        y[q] = acc;
     }
 }
 
-template <int N, int IDX, int NS>
-struct WfStages {
-    static __device__ __forceinline__ wfc *run(wfc *a, wfc *b, int nseq, const wfc *tw) {
+constexpr int wf_cmin(int a, int b) { return a < b ? a : b; }
+constexpr int wf_cmax(int a, int b) { return a > b ? a : b; }
+
+/* One in-place Stockham stage of radix R after radices of product NS over
+ * NSEQ sequences of length N in LDS, NT threads (same mathematics as
+ * wf_stockham_output).  Every thread first computes its outputs into
+ * registers, then the workgroup synchronises and writes them back, so one
+ * buffer suffices.  Ends with a barrier.
+ *
+ * Odd R: with a_r = x_r + x_{R-r}, b_r = x_r - x_{R-r} (twiddled inputs),
+ *   X_q     = x_0 + sum_{r<=H} a_r Re W^{rq} + i sum_{r<=H} b_r Im W^{rq},
+ *   X_{R-q} = x_0 + sum_{r<=H} a_r Re W^{rq} - i sum_{r<=H} b_r Im W^{rq},
+ * H = (R-1)/2, W = exp(sign 2 pi i / R) (uses W^{R-m} = conj W^m): a quarter of
+ * the multiplications of the direct sum.  QS threads share one butterfly, each
+ * streaming over the inputs and accumulating only its own pairs (q, R-q), so
+ * few registers are live and large radices still fill the workgroup.
+ * Even R (2, 4): one butterfly per thread, direct sum. */
+template <int N, int R, int NS, int NSEQ, int NT>
+__device__ __forceinline__ void wf_stage_ip(wfc *buf, const wfc *tw) {
+#pragma clang fp contract(fast)
+    constexpr int M = N / R;
+    constexpr int SC = N / (NS * R);
+    constexpr int WR = N / R;
+    constexpr int NB = NSEQ * M; /* butterflies */
+    if constexpr (R % 2 == 1) {
+        constexpr int H = (R - 1) / 2;
+        constexpr int QS = wf_cmax(1, wf_cmin(H, NT / NB)); /* threads per butterfly */
+        constexpr int QN = (H + QS - 1) / QS;               /* output pairs per thread */
+        constexpr int NW = NB * QS;
+        constexpr int PER = (NW + NT - 1) / NT;
+        wfc y0[PER], yp[PER][QN], ym[PER][QN];
+#pragma unroll
+        for (int pi = 0; pi < PER; pi++) {
+            const int t = threadIdx.x + pi * NT;
+            if (t < NW) {
+                const int bf = t / QS, u = t - bf * QS;
+                const int s = bf / M, j = bf - s * M;
+                const int kq = j % NS;
+                const wfc *src = buf + s * N + j;
+                const wfc x0 = src[0];
+                double c0r = x0.re, c0i = x0.im;
+                double cr[QN], ci[QN], sr[QN], si[QN];
+#pragma unroll
+                for (int qi = 0; qi < QN; qi++) {
+                    cr[qi] = x0.re;
+                    ci[qi] = x0.im;
+                    sr[qi] = 0.0;
+                    si[qi] = 0.0;
+                }
+                constexpr int UNR = (H <= 8) ? H : 4;
+#pragma unroll UNR
+                for (int r = 1; r <= H; r++) {
+                    wfc xa = src[r * M], xb = src[(R - r) * M];
+                    if constexpr (NS > 1) {
+                        xa = wf_cmul(xa, tw[kq * r * SC]);
+                        xb = wf_cmul(xb, tw[kq * (R - r) * SC]);
+                    }
+                    const double ar = xa.re + xb.re, ai = xa.im + xb.im;
+                    const double br = xa.re - xb.re, bi = xa.im - xb.im;
+                    c0r += ar;
+                    c0i += ai;
+#pragma unroll
+                    for (int qi = 0; qi < QN; qi++) {
+                        const int q = 1 + u + qi * QS;
+                        const int m = (r * q) % R;
+                        const int mm = (m <= H) ? m : R - m;
+                        const wfc W = tw[mm * WR];
+                        const double sw = (m <= H) ? W.im : -W.im;
+                        cr[qi] += ar * W.re;
+                        ci[qi] += ai * W.re;
+                        sr[qi] += bi * sw;
+                        si[qi] += br * sw;
+                    }
+                }
+                y0[pi].re = c0r;
+                y0[pi].im = c0i;
+#pragma unroll
+                for (int qi = 0; qi < QN; qi++) {
+                    yp[pi][qi].re = cr[qi] - sr[qi];
+                    yp[pi][qi].im = ci[qi] + si[qi];
+                    ym[pi][qi].re = cr[qi] + sr[qi];
+                    ym[pi][qi].im = ci[qi] - si[qi];
+                }
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int pi = 0; pi < PER; pi++) {
+            const int t = threadIdx.x + pi * NT;
+            if (t < NW) {
+                const int bf = t / QS, u = t - bf * QS;
+                const int s = bf / M, j = bf - s * M;
+                const int kq = j % NS;
+                wfc *dst = buf + s * N + (j / NS) * NS * R + kq;
+                if (u == 0) dst[0] = y0[pi];
+#pragma unroll
+                for (int qi = 0; qi < QN; qi++) {
+                    const int q = 1 + u + qi * QS;
+                    if (q <= H) {
+                        dst[q * NS] = yp[pi][qi];
+                        dst[(R - q) * NS] = ym[pi][qi];
+                    }
+                }
+            }
+        }
+        __syncthreads();
+    } else {
+        constexpr int PER = (NB + NT - 1) / NT;
+        wfc v[PER][R];
+#pragma unroll
+        for (int pi = 0; pi < PER; pi++) {
+            const int t = threadIdx.x + pi * NT;
+            if (t < NB) {
+                const int s = t / M, j = t - s * M;
+                const int kq = j % NS;
+                const wfc *src = buf + s * N + j;
+                wfc x[R];
+#pragma unroll
+                for (int r = 0; r < R; r++) x[r] = src[r * M];
+                if constexpr (NS > 1) {
+#pragma unroll
+                    for (int r = 1; r < R; r++) x[r] = wf_cmul(x[r], tw[kq * r * SC]);
+                }
+                wf_dft_generic<R, WR>(x, v[pi], tw);
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int pi = 0; pi < PER; pi++) {
+            const int t = threadIdx.x + pi * NT;
+            if (t < NB) {
+                const int s = t / M, j = t - s * M;
+                const int kq = j % NS;
+                wfc *dst = buf + s * N + (j / NS) * NS * R + kq;
+#pragma unroll
+                for (int q = 0; q < R; q++) dst[q * NS] = v[pi][q];
+            }
+        }
+        __syncthreads();
+    }
+}
+
+template <int N, int IDX, int NS, int NSEQ, int NT>
+struct WfStagesIP {
+    static __device__ __forceinline__ void run(wfc *buf, const wfc *tw) {
         constexpr int R = wf_radix_of(N, IDX);
-        if constexpr (R == 0) {
-            return a;
-        } else {
-            wf_stage_ct<N, R, NS>(a, b, nseq, tw);
-            __syncthreads();
-            return WfStages<N, IDX + 1, NS * R>::run(b, a, nseq, tw);
+        if constexpr (R != 0) {
+            wf_stage_ip<N, R, NS, NSEQ, NT>(buf, tw);
+            WfStagesIP<N, IDX + 1, NS * R, NSEQ, NT>::run(buf, tw);
         }
     }
 };
 
-/* Full transform of `nseq` sequences of compile-time length N held in LDS
- * buffer a (b is scratch); returns the buffer holding the result. */
-template <int N>
-__device__ __forceinline__ wfc *wf_fft_ct(wfc *a, wfc *b, int nseq, const wfc *tw) {
-    return WfStages<N, 0, 1>::run(a, b, nseq, tw);
+/* In-place transform of NSEQ sequences of compile-time length N held in LDS
+ * (natural order in and out), NT threads.  The caller synchronises after
+ * filling buf; the transform ends with a barrier. */
+template <int N, int NSEQ, int NT>
+__device__ __forceinline__ void wf_fft_ip(wfc *buf, const wfc *tw) {
+    WfStagesIP<N, 0, 1, NSEQ, NT>::run(buf, tw);
 }
 #endif
 

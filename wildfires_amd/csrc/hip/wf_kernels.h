@@ -71,22 +71,34 @@ hipError_t wf_launch_to_reference(const double *src, double *dst, int Nx, int Ny
 
 /* Device-side control of the variable-density pressure iteration. */
 struct WfIterCtl {
-    unsigned long long norm_bits; /* max |p_new - p_old| of the current solve, as ordered bits */
-    int done;                     /* 1 once converged or out of iterations */
-    int iters;                    /* the count the reference reports */
-    int solves;                   /* solves performed */
-    double err;                   /* last max |p_new - p_old| */
+    int done;   /* 1 once converged or out of iterations */
+    int iters;  /* the count the reference reports */
+    int solves; /* solves performed */
+    double err; /* last max |p_new - p_old| */
 };
+
+/* Row pairs per workgroup in the x-transforms and kx columns per workgroup in
+ * the y-transform, for a transform length n. */
+__host__ __device__ constexpr int wf_pairs_for(int n) { return (n <= 300) ? 4 : 2; }
+#define WF_FY_COLS 4
+/* z-levels per lane in the chunked tridiagonal solve and the largest nz */
+#define WF_THOMAS_MAX_CHUNKS 16
+#define WF_NZ_MAX (16 * WF_THOMAS_MAX_CHUNKS)
 
 struct WfSpectral {
     int nx, ny, nz; /* unique nodes per direction: Nx-1, Ny-1, Nz-1 */
     int hx;         /* nx/2 + 1 retained x-wavenumbers */
+    int ldj;        /* ny rounded up to a multiple of 8: stride of one kx column */
     int fx_pairs;   /* row pairs per workgroup in the x-transforms */
     int fy_cols;    /* kx columns per workgroup in the y-transform */
-    double2 *F;     /* [nz][ny][hx] spectral working buffer */
-    double2 *Ptop;  /* [ny][hx] spectrum of the top pressure plane */
+    int ngroups;    /* row-pair groups per slab in the x-transforms */
+    int npmax;      /* workgroups of the inverse x-transform (partial maxima) */
+    double2 *F;     /* [nz][hx][ldj] spectral working buffer (ky fastest) */
+    double2 *Ptop;  /* [hx][ldj] spectrum of the top pressure plane */
     double *A;      /* Nx*Ny*Nz: p-independent part of the Poisson RHS */
-    const double *U;     /* [nz][ny][hx]: Thomas pivots u_k */
+    const double *L;     /* [nz][hx][ldj]: forward multipliers a_{k-1}/u_{k-1} (k >= 1) */
+    const double *RU;    /* [nz][hx][ldj]: 1/u_k, u_k the Thomas pivots */
+    double *pmax;        /* npmax partial maxima of |p_new - p_old| */
     const double *kx2;   /* hx: kx^2 of the retained x-wavenumbers */
     const double *ky2;   /* ny */
     const double *ta;    /* nz-1: sub-diagonal a_k (k-only) */
@@ -103,16 +115,20 @@ struct WfSpectral {
     WfIterCtl *ctl;
 };
 
+/* Forms of the x-transform input: the Poisson RHS of a later iteration (stored
+ * p-independent part + pressure term), of the first iteration (also stores
+ * that part), of the constant-density solve, and the top pressure plane. */
+enum { WF_F_ITER = 0, WF_F_FIRST = 1, WF_F_CONST = 2, WF_F_TOP = 3 };
+
 hipError_t wf_launch_pressure_rhs_fx(const WfConst &c, const WfSpectral &sp, const WfState &Ys, const WfState &Yn,
-                                     const double *p, int first_iter, hipStream_t s);
-hipError_t wf_launch_ptop_fx(const WfConst &c, const WfSpectral &sp, const double *p, hipStream_t s);
-hipError_t wf_launch_fy(const WfSpectral &sp, double2 *buf, int slab0, int nslabs, int sign, int guarded,
-                        hipStream_t s);
+                                     const double *p, int mode, hipStream_t s);
+hipError_t wf_launch_fy(const WfSpectral &sp, double2 *buf, long slab_stride, int slab0, int nslabs, int sign,
+                        int guarded, hipStream_t s);
 hipError_t wf_launch_density(const WfConst &c, const double *T, double *rho, hipStream_t s);
 hipError_t wf_launch_iter_reset(WfIterCtl *ctl, hipStream_t s);
-hipError_t wf_launch_iter_check(WfIterCtl *ctl, double tol, int m, int max_iter, hipStream_t s);
+hipError_t wf_launch_iter_check(const WfSpectral &sp, double tol, int m, int max_iter, hipStream_t s);
 hipError_t wf_launch_thomas(const WfSpectral &sp, hipStream_t s);
-hipError_t wf_launch_pivots(const WfSpectral &sp, double *U, hipStream_t s);
+hipError_t wf_launch_pivots(const WfSpectral &sp, double *L, double *RU, hipStream_t s);
 hipError_t wf_launch_inverse_fx(const WfConst &c, const WfSpectral &sp, double *p, hipStream_t s);
 
 /* Temperature source applied at the end of a step (src/cu/functions.cu:325-403). */

@@ -9,10 +9,14 @@
  * y-transform runs on that half spectrum, and the Thomas solve runs on the
  * nx/2+1 retained kx bins only; the discarded half is the complex conjugate.
  *
- *   P1 k_rhs_fx   : f (or the top pressure plane) -> x half-spectrum   [per row pair]
- *   P2 k_fy       : y-transform of the half spectrum (forward/inverse) [per slab, kx block]
- *   P3 k_thomas   : tridiagonal solve per (kx, ky), pivots precomputed [per system]
- *   P5 k_inv_fx   : inverse x-transform, p, periodic duplicates, |dp|max [per row pair]
+ *   P1 k_rhs_fx   : f (or the top pressure plane) -> x half-spectrum   [row-pair group, slab]
+ *   P2 k_fy       : y-transform of WF_FY_COLS kx columns, one slab      [forward and inverse]
+ *   P3 k_thomas   : tridiagonal solve, 64 (kx, ky) systems per workgroup, z split over its waves
+ *   P5 k_inv_fx   : inverse x-transform, p, periodic duplicates, max |dp| per workgroup
+ *
+ * Spectral layout: F[k][kx][ky] with ky fastest (column stride ldj), so a
+ * y-transform reads whole contiguous columns and the tridiagonal solve reads
+ * 64 consecutive ky per wavefront.
  *
  * The variable-density fixed-point loop is controlled on the device
  * (WfIterCtl): every kernel of a solve returns at once after convergence, so
@@ -21,6 +25,8 @@
 #include "wf_common.h"
 #include "wf_fft.h"
 #include "wf_kernels.h"
+
+#define WF_NT 256
 
 /* One full FFT of `nseq` sequences of run-time length n held in LDS
  * (a -> b -> a ...), for lengths without a compile-time plan.  Returns the
@@ -41,6 +47,43 @@ __device__ wfc *wf_lds_fft(wfc *a, wfc *b, int n, int nseq, const int *rad, cons
         Ns *= R;
     }
     return a;
+}
+
+/* Transform of the NSEQ sequences in buf (compile-time plan when N > 0, else
+ * the run-time radices with scratch buffer bufb).  Returns the result buffer. */
+template <int N, int NSEQ>
+__device__ __forceinline__ wfc *wf_transform(wfc *buf, wfc *bufb, int n, int nseq, const int *rad, const wfc *tw) {
+    if constexpr (N > 0) {
+        wf_fft_ip<N, NSEQ, WF_NT>(buf, tw);
+        return buf;
+    } else {
+        return wf_lds_fft(buf, bufb, n, nseq, rad, tw);
+    }
+}
+
+/* Workgroups are dispatched round-robin over the 8 XCDs (each with its own
+ * L2); this maps the hardware block index to a logical tile so that each XCD
+ * works on one contiguous range of tiles and neighbouring tiles share an L2. */
+__device__ __forceinline__ int wf_xcd_tile(int b, int nblocks) {
+    const int per = nblocks >> 3, rem = nblocks & 7;
+    const int x = b & 7, idx = b >> 3;
+    return x * per + (x < rem ? x : rem) + idx;
+}
+
+/* Maximum of v over the workgroup, written by thread 0 to *out. */
+__device__ __forceinline__ void wf_block_max_store(double v, double *out) {
+    __shared__ double red[WF_NT / 64];
+    for (int off = 32; off > 0; off >>= 1) {
+        double o = __shfl_xor(v, off, 64);
+        v = (o > v) ? o : v;
+    }
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double m = red[0];
+        for (int w = 1; w < (int)(blockDim.x >> 6); w++) m = (red[w] > m) ? red[w] : m;
+        *out = m;
+    }
 }
 
 /* p-independent part of f at node (i,j,k), 1 <= k <= Nz-2
@@ -97,7 +140,8 @@ __device__ double wf_f_fixed(const WfConst &c, const WfState &Ys, const WfState 
 
 /* (grad rho . grad p) / rho at node (i,j,k), 1 <= k <= Nz-2, with rho = rho^n
  * the density of y_n (src/cu/pressure.cu:163-197, 269-282, 319). */
-__device__ double wf_f_pgrad(const WfConst &c, const double *rho_n, const double *p, int i, int j, int k) {
+__device__ __forceinline__ double wf_f_pgrad(const WfConst &c, const double *rho_n, const double *p, int i, int j,
+                                             int k) {
     const int Nx = c.Nx, Ny = c.Ny;
     const long P = c.plane;
     const double *z = c.z;
@@ -121,22 +165,27 @@ __global__ void k_density(WfConst c, const double *T, double *rho) {
 }
 
 __global__ void k_iter_reset(WfIterCtl *ctl) {
-    ctl->norm_bits = 0ull;
     ctl->done = 0;
     ctl->iters = 0;
     ctl->solves = 0;
     ctl->err = 0.0;
 }
 
-/* After solve m: err = max|p_new - p_old|, stop once err <= tol or after
- * max_iter solves (src/cu/pressure.cu:564-574); iters is the count the
- * reference reports (the index of the converging solve, else max_iter). */
-__global__ void k_iter_check(WfIterCtl *ctl, double tol, int m, int max_iter) {
+/* After solve m: err = max|p_new - p_old| (maximum of the inverse transform's
+ * per-workgroup maxima), stop once err <= tol or after max_iter solves
+ * (src/cu/pressure.cu:564-574); iters is the count the reference reports (the
+ * index of the converging solve, else max_iter). */
+__global__ __launch_bounds__(WF_NT) void k_iter_check(WfIterCtl *ctl, const double *pmax, int npmax, double tol,
+                                                      int m, int max_iter) {
     if (ctl->done) return;
-    double err = __longlong_as_double((long long)ctl->norm_bits);
+    double v = 0.0;
+    for (int t = threadIdx.x; t < npmax; t += blockDim.x) v = (pmax[t] > v) ? pmax[t] : v;
+    __shared__ double err;
+    wf_block_max_store(v, &err);
+    __syncthreads();
+    if (threadIdx.x != 0) return;
     ctl->err = err;
     ctl->solves = m + 1;
-    ctl->norm_bits = 0ull;
     if (err <= tol) {
         ctl->done = 1;
         ctl->iters = m;
@@ -146,248 +195,351 @@ __global__ void k_iter_check(WfIterCtl *ctl, double tol, int m, int max_iter) {
     }
 }
 
-/* Wavefront max of a non-negative double, folded into *bits (IEEE order of
- * non-negative doubles equals the unsigned order of their bit patterns). */
-__device__ void wf_atomic_max_nonneg(unsigned long long *bits, double v) {
-    unsigned long long b = __double_as_longlong(v);
-    for (int off = 32; off > 0; off >>= 1) {
-        unsigned long long o = __shfl_xor(b, off, 64);
-        b = (o > b) ? o : b;
+/* Input value of the x-transform at (i, j, k) for form MODE. */
+template <int MODE>
+__device__ __forceinline__ double wf_fx_input(const WfConst &c, const WfSpectral &sp, const WfState &Ys,
+                                              const WfState &Yn, const double *p, int i, int j, int k) {
+    long q = wf_q(c, i, j, k);
+    if constexpr (MODE == WF_F_TOP) {
+        return p[q];
+    } else if constexpr (MODE == WF_F_CONST) {
+        return wf_f_fixed(c, Ys, Yn, i, j, k);
+    } else if constexpr (MODE == WF_F_FIRST) {
+        double f = wf_f_fixed(c, Ys, Yn, i, j, k);
+        sp.A[q] = f;
+        return f + wf_f_pgrad(c, sp.rho, p, i, j, k);
+    } else {
+        return sp.A[q] + wf_f_pgrad(c, sp.rho, p, i, j, k);
     }
-    if ((threadIdx.x & 63) == 0) atomicMax(bits, b);
 }
 
-/* P1: f on slab k (1..nz-1) for sp.fx_pairs row pairs, forward x half-spectrum.
- * source 0: Poisson RHS (first_iter: also stores its p-independent part in sp.A);
- * source 1: the top pressure plane into sp.Ptop.  NX > 0 selects the
- * compile-time plan of that length. */
-template <int NX>
-__global__ __launch_bounds__(256) void k_rhs_fx(WfConst c, WfSpectral sp, WfState Ys, WfState Yn, const double *p,
-                                                int first_iter, int source) {
-    if (source == 0 && sp.ctl->done) return;
+/* P1: f on slab k (1..nz-1; the top plane for WF_F_TOP) for one group of row
+ * pairs (rows ja = 2m, jb = 2m+1 packed as ja + i jb), forward x
+ * half-spectrum into F[k][kx][j].  NX > 0 selects the compile-time plan. */
+template <int NX, int MODE>
+__global__ __launch_bounds__(WF_NT) void k_rhs_fx(WfConst c, WfSpectral sp, WfState Ys, WfState Yn,
+                                                  const double *p) {
+    if (MODE != WF_F_TOP && sp.ctl->done) return;
     extern __shared__ wfc smem[];
     __shared__ int rad[WF_MAX_RADICES];
+    constexpr int PAIRS = (NX > 0) ? wf_pairs_for(NX) : 0;
     const int nx = (NX > 0) ? NX : sp.nx;
-    const int ny = sp.ny, hx = sp.hx, pairs = sp.fx_pairs;
+    const int pairs = (NX > 0) ? PAIRS : sp.fx_pairs;
+    const int ny = sp.ny, hx = sp.hx, ldj = sp.ldj;
     wfc *tw = smem, *bufa = smem + nx, *bufb = bufa + pairs * nx;
-    int k = (source == 1) ? c.Nz - 1 : 1 + blockIdx.y;
-    int jpair0 = blockIdx.x * pairs;
+    const int tile = wf_xcd_tile(blockIdx.x, gridDim.x);
+    const int g = tile % sp.ngroups;
+    const int k = (MODE == WF_F_TOP) ? c.Nz - 1 : 1 + tile / sp.ngroups;
+    const int j0 = 2 * g * pairs;
     for (int t = threadIdx.x; t < nx; t += blockDim.x) tw[t] = *(const wfc *)&sp.twx[t];
     if (NX == 0 && threadIdx.x < WF_MAX_RADICES) rad[threadIdx.x] = sp.radx[threadIdx.x];
-    for (int t = threadIdx.x; t < pairs * nx; t += blockDim.x) {
-        int s = t / nx, i = t - s * nx;
-        int ja = 2 * (jpair0 + s), jb = ja + 1;
-        double fa = 0.0, fb = 0.0;
-        if (ja < ny) {
-            if (source == 1) {
-                fa = p[wf_q(c, i, ja, k)];
-                if (jb < ny) fb = p[wf_q(c, i, jb, k)];
-            } else {
-                long qa = wf_q(c, i, ja, k);
-                if (first_iter || c.variable_density == 0) {
-                    fa = wf_f_fixed(c, Ys, Yn, i, ja, k);
-                    if (c.variable_density) sp.A[qa] = fa;
-                } else {
-                    fa = sp.A[qa];
-                }
-                if (c.variable_density) fa = fa + wf_f_pgrad(c, sp.rho, p, i, ja, k);
-                if (jb < ny) {
-                    long qb = wf_q(c, i, jb, k);
-                    if (first_iter || c.variable_density == 0) {
-                        fb = wf_f_fixed(c, Ys, Yn, i, jb, k);
-                        if (c.variable_density) sp.A[qb] = fb;
-                    } else {
-                        fb = sp.A[qb];
-                    }
-                    if (c.variable_density) fb = fb + wf_f_pgrad(c, sp.rho, p, i, jb, k);
-                }
+    if constexpr (NX > 0) {
+        constexpr int PER = (PAIRS * NX + WF_NT - 1) / WF_NT;
+#pragma unroll
+        for (int pi = 0; pi < PER; pi++) {
+            const int t = threadIdx.x + pi * WF_NT;
+            if (t < PAIRS * NX) {
+                const int s = t / NX, i = t - s * NX;
+                const int ja = j0 + 2 * s, jb = ja + 1;
+                double fa = 0.0, fb = 0.0;
+                if (ja < ny) fa = wf_fx_input<MODE>(c, sp, Ys, Yn, p, i, ja, k);
+                if (jb < ny) fb = wf_fx_input<MODE>(c, sp, Ys, Yn, p, i, jb, k);
+                bufa[t].re = fa;
+                bufa[t].im = fb;
             }
         }
-        bufa[s * nx + i].re = fa;
-        bufa[s * nx + i].im = fb;
+    } else {
+        for (int t = threadIdx.x; t < pairs * nx; t += blockDim.x) {
+            const int s = t / nx, i = t - s * nx;
+            const int ja = j0 + 2 * s, jb = ja + 1;
+            double fa = 0.0, fb = 0.0;
+            if (ja < ny) fa = wf_fx_input<MODE>(c, sp, Ys, Yn, p, i, ja, k);
+            if (jb < ny) fb = wf_fx_input<MODE>(c, sp, Ys, Yn, p, i, jb, k);
+            bufa[t].re = fa;
+            bufa[t].im = fb;
+        }
     }
     __syncthreads();
-    wfc *res;
-    if constexpr (NX > 0)
-        res = wf_fft_ct<NX>(bufa, bufb, pairs, tw);
-    else
-        res = wf_lds_fft(bufa, bufb, nx, pairs, rad, tw);
-    double2 *dst = (source == 1) ? sp.Ptop : sp.F + (long)k * ny * hx;
-    for (int t = threadIdx.x; t < pairs * hx; t += blockDim.x) {
-        int s = t / hx, r = t - s * hx;
-        int ja = 2 * (jpair0 + s), jb = ja + 1;
-        if (ja >= ny) continue;
-        int rm = (r == 0) ? 0 : nx - r;
+    wfc *res = wf_transform<NX, PAIRS>(bufa, bufb, nx, pairs, rad, tw);
+    double2 *dst = (MODE == WF_F_TOP) ? sp.Ptop : sp.F + (long)k * hx * ldj;
+    const int rows = 2 * pairs;
+    for (int t = threadIdx.x; t < rows * hx; t += blockDim.x) {
+        const int jl = t % rows, r = t / rows;
+        const int j = j0 + jl;
+        if (j >= ny) continue;
+        const int s = jl >> 1;
+        const int rm = (r == 0) ? 0 : nx - r;
         wfc A, B;
         wf_unpack_pair(res[s * nx + r], res[s * nx + rm], &A, &B);
-        dst[(long)ja * hx + r] = make_double2(A.re, A.im);
-        if (jb < ny) dst[(long)jb * hx + r] = make_double2(B.re, B.im);
+        dst[(long)r * ldj + j] = (jl & 1) ? make_double2(B.re, B.im) : make_double2(A.re, A.im);
     }
 }
 
-/* P2/P4: y-transform of sp.fy_cols kx columns over all ky, one slab. */
+/* P2/P4: y-transform of WF_FY_COLS kx columns (each contiguous over ky) of
+ * slab slab0 + blockIdx.y. */
 template <int NY>
-__global__ __launch_bounds__(256) void k_fy(WfSpectral sp, double2 *buf, int slab0, int sign, int guarded) {
+__global__ __launch_bounds__(WF_NT) void k_fy(WfSpectral sp, double2 *buf, long slab_stride, int slab0, int sign,
+                                              int guarded) {
     if (guarded && sp.ctl->done) return;
     extern __shared__ wfc smem[];
     __shared__ int rad[WF_MAX_RADICES];
     const int ny = (NY > 0) ? NY : sp.ny;
-    const int hx = sp.hx, cols = sp.fy_cols;
-    wfc *tw = smem, *bufa = smem + ny, *bufb = bufa + cols * ny;
-    int r0 = blockIdx.x * cols;
-    long slab = (long)(slab0 + blockIdx.y) * ny * hx;
+    const int hx = sp.hx, ldj = sp.ldj;
+    constexpr int COLS = WF_FY_COLS;
+    wfc *tw = smem, *bufa = smem + ny, *bufb = bufa + COLS * ny;
+    const int r0 = blockIdx.x * COLS;
+    double2 *base = buf + (long)(slab0 + blockIdx.y) * slab_stride;
     for (int t = threadIdx.x; t < ny; t += blockDim.x) {
         double2 w = sp.twy[t];
         tw[t].re = w.x;
         tw[t].im = (sign > 0) ? -w.y : w.y;
     }
     if (NY == 0 && threadIdx.x < WF_MAX_RADICES) rad[threadIdx.x] = sp.rady[threadIdx.x];
-    for (int t = threadIdx.x; t < cols * ny; t += blockDim.x) {
-        int cc = t % cols, jj = t / cols;
-        int r = r0 + cc;
-        wfc v = {0.0, 0.0};
-        if (r < hx) {
-            double2 x = buf[slab + (long)jj * hx + r];
-            v.re = x.x;
-            v.im = x.y;
+    if constexpr (NY > 0) {
+        constexpr int PER = (COLS * NY + WF_NT - 1) / WF_NT;
+        double2 v[PER];
+#pragma unroll
+        for (int pi = 0; pi < PER; pi++) {
+            const int t = threadIdx.x + pi * WF_NT;
+            const int cc = t / NY, jj = t - cc * NY;
+            v[pi] = make_double2(0.0, 0.0);
+            if (t < COLS * NY && r0 + cc < hx) v[pi] = base[(long)(r0 + cc) * ldj + jj];
         }
-        bufa[cc * ny + jj] = v;
+#pragma unroll
+        for (int pi = 0; pi < PER; pi++) {
+            const int t = threadIdx.x + pi * WF_NT;
+            if (t < COLS * NY) {
+                bufa[t].re = v[pi].x;
+                bufa[t].im = v[pi].y;
+            }
+        }
+    } else {
+        for (int t = threadIdx.x; t < COLS * ny; t += blockDim.x) {
+            const int cc = t / ny, jj = t - cc * ny;
+            wfc v = {0.0, 0.0};
+            if (r0 + cc < hx) {
+                double2 x = base[(long)(r0 + cc) * ldj + jj];
+                v.re = x.x;
+                v.im = x.y;
+            }
+            bufa[t] = v;
+        }
     }
     __syncthreads();
-    wfc *res;
-    if constexpr (NY > 0)
-        res = wf_fft_ct<NY>(bufa, bufb, cols, tw);
-    else
-        res = wf_lds_fft(bufa, bufb, ny, cols, rad, tw);
-    for (int t = threadIdx.x; t < cols * ny; t += blockDim.x) {
-        int cc = t % cols, jj = t / cols;
-        int r = r0 + cc;
-        if (r < hx) buf[slab + (long)jj * hx + r] = make_double2(res[cc * ny + jj].re, res[cc * ny + jj].im);
+    wfc *res = wf_transform<NY, COLS>(bufa, bufb, ny, COLS, rad, tw);
+    for (int t = threadIdx.x; t < COLS * ny; t += blockDim.x) {
+        const int cc = t / ny, jj = t - cc * ny;
+        if (r0 + cc < hx) base[(long)(r0 + cc) * ldj + jj] = make_double2(res[t].re, res[t].im);
     }
 }
 
-/* Thomas pivots u_k of every (kx, ky) system (src/cu/pressure.cu:429-433):
+/* Thomas pivots of every (kx, ky) system (src/cu/pressure.cu:429-433):
  * u_0 = b_0, u_k = b_k - (a_{k-1}/u_{k-1}) c_{k-1}.  They do not change between
- * steps, so they are formed once. */
-__global__ void k_pivots(WfSpectral sp, double *U) {
-    int r = blockIdx.x * blockDim.x + threadIdx.x;
-    int s = blockIdx.y;
-    if (r >= sp.hx) return;
+ * steps, so they are formed once, stored as the forward multipliers
+ * L_k = a_{k-1}/u_{k-1} and the reciprocals 1/u_k. */
+__global__ void k_pivots(WfSpectral sp, double *L, double *RU) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    const int r = blockIdx.y;
+    if (j >= sp.ny) return;
     const int nz = sp.nz;
-    const long slab = (long)sp.ny * sp.hx;
-    long o = (long)s * sp.hx + r;
-    double K2 = sp.kx2[r] + sp.ky2[s];
+    const long slab = (long)sp.hx * sp.ldj;
+    const long o = (long)r * sp.ldj + j;
+    double K2 = sp.kx2[r] + sp.ky2[j];
     double c0 = sp.c0a - 0.5 * sp.dz0 * K2 - sp.c0b;
     double u = sp.b0;
-    U[o] = u;
+    L[o] = 0.0;
+    RU[o] = 1.0 / u;
     for (int k = 1; k < nz; k++) {
         double ck1 = (k == 1) ? c0 : sp.tc[k - 1];
         double bk = (k == nz - 1) ? -(K2 + sp.beta_top) : -(K2 + sp.beta[k]);
         double l = sp.ta[k - 1] / u;
         u = bk - l * ck1;
-        U[o + k * slab] = u;
+        L[o + k * slab] = l;
+        RU[o + k * slab] = 1.0 / u;
     }
 }
 
-/* P3: forward elimination and back substitution per (kx, ky), in place in sp.F
- * (src/cu/pressure.cu:418-443 with the right-hand side of 473-504). */
-__global__ void k_thomas(WfSpectral sp) {
+/* P3: tridiagonal solve for 64 consecutive ky of one kx column per workgroup,
+ * in place in F (src/cu/pressure.cu:418-443 with the right-hand side of
+ * 473-504):  y_0 = d_0, y_k = d_k - L_k y_{k-1};  x_{nz-1} = y_{nz-1}/u_{nz-1},
+ * x_k = (y_k - c_k x_{k+1})/u_k.
+ * Wave w owns levels [w LC, w LC + LC) of all 64 systems and keeps them in
+ * registers.  Both recurrences are affine in the value entering the chunk, so
+ * each wave runs them from a zero entry value while carrying the coefficient
+ * of the entry value (products of -L_k, resp. of -c_k/u_k); the waves then
+ * combine their chunk ends through LDS in order and add the correction. */
+template <int LC>
+__global__ __launch_bounds__(64 * WF_THOMAS_MAX_CHUNKS) void k_thomas(WfSpectral sp) {
     if (sp.ctl->done) return;
-    int r = blockIdx.x * blockDim.x + threadIdx.x;
-    int s = blockIdx.y;
-    if (r >= sp.hx) return;
+    __shared__ double2 sv[WF_THOMAS_MAX_CHUNKS][64];
+    __shared__ double sg[WF_THOMAS_MAX_CHUNKS][64];
+    const int lane = threadIdx.x & 63;
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int C = blockDim.x >> 6;
+    const int j = blockIdx.x * 64 + lane, r = blockIdx.y;
     const int nz = sp.nz;
-    const long slab = (long)sp.ny * sp.hx;
-    long o = (long)s * sp.hx + r;
-    double2 *F = sp.F;
-    double K2 = sp.kx2[r] + sp.ky2[s];
-    double c0 = sp.c0a - 0.5 * sp.dz0 * K2 - sp.c0b;
-    /* d_0 = (dz_0/2) f^_1 */
-    double2 f1 = F[o + slab];
-    double yr = 0.5 * sp.dz0 * f1.x, yi = 0.5 * sp.dz0 * f1.y;
-    F[o] = make_double2(yr, yi);
-    double u_prev = sp.U[o];
-    for (int k = 1; k < nz; k++) {
-        double2 d = F[o + k * slab];
-        if (k == nz - 1) {
-            double2 pt = sp.Ptop[o];
-            d.x = d.x - pt.x / sp.top_div;
-            d.y = d.y - pt.y / sp.top_div;
+    const bool ok = j < sp.ny;
+    const long slab = (long)sp.hx * sp.ldj;
+    const long o = (long)r * sp.ldj + (ok ? j : 0);
+    const int k0 = w * LC;
+    double2 y[LC];
+    double g[LC], ru[LC];
+#pragma unroll
+    for (int t = 0; t < LC; t++) {
+        const int k = k0 + t;
+        y[t] = make_double2(0.0, 0.0);
+        g[t] = 0.0;
+        ru[t] = 0.0;
+        if (ok && k < nz) {
+            if (k == 0) {
+                double2 f1 = sp.F[o + slab];
+                y[t] = make_double2(0.5 * sp.dz0 * f1.x, 0.5 * sp.dz0 * f1.y);
+            } else {
+                double2 d = sp.F[o + k * slab];
+                if (k == nz - 1) {
+                    double2 pt = sp.Ptop[o];
+                    d.x = d.x - pt.x / sp.top_div;
+                    d.y = d.y - pt.y / sp.top_div;
+                }
+                y[t] = d;
+                g[t] = sp.L[o + k * slab];
+            }
+            ru[t] = sp.RU[o + k * slab];
         }
-        double l = sp.ta[k - 1] / u_prev;
-        yr = d.x - l * yr;
-        yi = d.y - l * yi;
-        F[o + k * slab] = make_double2(yr, yi);
-        u_prev = sp.U[o + k * slab];
     }
-    double xr = yr / u_prev, xi = yi / u_prev;
-    F[o + (nz - 1) * slab] = make_double2(xr, xi);
-    for (int k = nz - 2; k >= 0; k--) {
-        double2 y = F[o + k * slab];
-        double ck = (k == 0) ? c0 : sp.tc[k];
-        double uk = sp.U[o + k * slab];
-        xr = (y.x - ck * xr) / uk;
-        xi = (y.y - ck * xi) / uk;
-        F[o + k * slab] = make_double2(xr, xi);
+    /* forward elimination, entry value 0, coefficient G of the entry value */
+    double2 acc = make_double2(0.0, 0.0);
+    double G = 1.0;
+#pragma unroll
+    for (int t = 0; t < LC; t++) {
+        const int k = k0 + t;
+        if (k == 0) {
+            acc = y[0];
+            G = 0.0;
+        } else {
+            const double m = g[t];
+            acc.x = y[t].x - m * acc.x;
+            acc.y = y[t].y - m * acc.y;
+            G = -m * G;
+            y[t] = acc;
+        }
+        g[t] = G;
+    }
+    sv[w][lane] = acc;
+    sg[w][lane] = G;
+    __syncthreads();
+    double2 Y = make_double2(0.0, 0.0);
+    for (int q = 0; q < w; q++) {
+        const double2 e = sv[q][lane];
+        const double gq = sg[q][lane];
+        Y.x = e.x + gq * Y.x;
+        Y.y = e.y + gq * Y.y;
+    }
+#pragma unroll
+    for (int t = 0; t < LC; t++) {
+        y[t].x = y[t].x + g[t] * Y.x;
+        y[t].y = y[t].y + g[t] * Y.y;
+    }
+    __syncthreads();
+    /* back substitution, entry value x_{k0+LC} = 0, coefficient H */
+    const double K2 = ok ? sp.kx2[r] + sp.ky2[j] : 0.0;
+    const double c0 = sp.c0a - 0.5 * sp.dz0 * K2 - sp.c0b;
+    double2 xa = make_double2(0.0, 0.0);
+    double H = 1.0;
+#pragma unroll
+    for (int t = LC - 1; t >= 0; t--) {
+        const int k = k0 + t;
+        if (k >= nz) continue;
+        if (k == nz - 1) {
+            xa.x = y[t].x * ru[t];
+            xa.y = y[t].y * ru[t];
+            H = 0.0;
+        } else {
+            const double ck = (k == 0) ? c0 : sp.tc[k];
+            xa.x = (y[t].x - ck * xa.x) * ru[t];
+            xa.y = (y[t].y - ck * xa.y) * ru[t];
+            H = (-ck * ru[t]) * H;
+        }
+        y[t] = xa;
+        g[t] = H;
+    }
+    sv[w][lane] = xa;
+    sg[w][lane] = H;
+    __syncthreads();
+    double2 X = make_double2(0.0, 0.0);
+    for (int q = C - 1; q > w; q--) {
+        const double2 e = sv[q][lane];
+        const double hq = sg[q][lane];
+        X.x = e.x + hq * X.x;
+        X.y = e.y + hq * X.y;
+    }
+    if (!ok) return;
+#pragma unroll
+    for (int t = 0; t < LC; t++) {
+        const int k = k0 + t;
+        if (k < nz) sp.F[o + k * slab] = make_double2(y[t].x + g[t] * X.x, y[t].y + g[t] * X.y);
     }
 }
 
-/* P5: inverse x-transform of sp.fx_pairs row pairs on slab k (0..nz-1):
+/* P5: inverse x-transform of one row-pair group of slab k (0..nz-1):
  * p = Re(.)/(nx ny) on unique nodes, duplicate planes i = Nx-1 and j = Ny-1 set
- * from their images [Q6], and the running max |p_new - p_old| into the
- * iteration control.  Workgroup row blockIdx.y = nz refreshes the periodic
- * copies on the top plane, which the solve never writes on unique nodes. */
+ * from their images [Q6], and max |p_new - p_old| of the workgroup into
+ * sp.pmax[blockIdx.x].  The last workgroup refreshes the periodic copies on the
+ * top plane, which the solve never writes on unique nodes. */
 template <int NX>
-__global__ __launch_bounds__(256) void k_inv_fx(WfConst c, WfSpectral sp, double *p) {
+__global__ __launch_bounds__(WF_NT) void k_inv_fx(WfConst c, WfSpectral sp, double *p) {
     if (sp.ctl->done) return;
     extern __shared__ wfc smem[];
     __shared__ int rad[WF_MAX_RADICES];
+    constexpr int PAIRS = (NX > 0) ? wf_pairs_for(NX) : 0;
     const int nx = (NX > 0) ? NX : sp.nx;
-    const int ny = sp.ny, hx = sp.hx, Nx = c.Nx, pairs = sp.fx_pairs;
-    int k = blockIdx.y;
+    const int pairs = (NX > 0) ? PAIRS : sp.fx_pairs;
+    const int ny = sp.ny, hx = sp.hx, ldj = sp.ldj, Nx = c.Nx;
     double dmax = 0.0;
-    if (k == sp.nz) {
-        if (blockIdx.x == 0) {
-            int kt = c.Nz - 1;
-            for (int t = threadIdx.x; t < c.Ny + c.Nx - 1; t += blockDim.x) {
-                long qd, qs;
-                if (t < c.Ny) { /* i = Nx-1 from i = 0 */
-                    qd = wf_q(c, c.Nx - 1, t, kt);
-                    qs = wf_q(c, 0, (t == c.Ny - 1) ? 0 : t, kt);
-                } else { /* j = Ny-1 from j = 0 */
-                    qd = wf_q(c, t - c.Ny, c.Ny - 1, kt);
-                    qs = wf_q(c, t - c.Ny, 0, kt);
-                }
-                double v = p[qs];
-                double d = fabs(v - p[qd]);
-                dmax = (d > dmax) ? d : dmax;
-                p[qd] = v;
+    if (blockIdx.x == gridDim.x - 1) {
+        const int kt = c.Nz - 1;
+        for (int t = threadIdx.x; t < c.Ny + c.Nx - 1; t += blockDim.x) {
+            long qd, qs;
+            if (t < c.Ny) { /* i = Nx-1 from i = 0 */
+                qd = wf_q(c, c.Nx - 1, t, kt);
+                qs = wf_q(c, 0, (t == c.Ny - 1) ? 0 : t, kt);
+            } else { /* j = Ny-1 from j = 0 */
+                qd = wf_q(c, t - c.Ny, c.Ny - 1, kt);
+                qs = wf_q(c, t - c.Ny, 0, kt);
             }
+            double v = p[qs];
+            double d = fabs(v - p[qd]);
+            dmax = (d > dmax) ? d : dmax;
+            p[qd] = v;
         }
-        wf_atomic_max_nonneg(&sp.ctl->norm_bits, dmax);
+        wf_block_max_store(dmax, &sp.pmax[blockIdx.x]);
         return;
     }
     wfc *tw = smem, *bufa = smem + nx, *bufb = bufa + pairs * nx;
-    int jpair0 = blockIdx.x * pairs;
-    const double2 *src = sp.F + (long)k * ny * hx;
+    const int tile = wf_xcd_tile(blockIdx.x, gridDim.x - 1);
+    const int g = tile % sp.ngroups, k = tile / sp.ngroups;
+    const int j0 = 2 * g * pairs;
+    const double2 *src = sp.F + (long)k * hx * ldj;
     for (int t = threadIdx.x; t < nx; t += blockDim.x) {
         double2 w = sp.twx[t];
         tw[t].re = w.x;
         tw[t].im = -w.y;
     }
     if (NX == 0 && threadIdx.x < WF_MAX_RADICES) rad[threadIdx.x] = sp.radx[threadIdx.x];
+    /* thread t -> pair s = t % pairs, bin r = t / pairs: consecutive threads read
+     * consecutive ky of one kx column */
     for (int t = threadIdx.x; t < pairs * nx; t += blockDim.x) {
-        int s = t / nx, r = t - s * nx;
-        int ja = 2 * (jpair0 + s), jb = ja + 1;
+        const int s = t % pairs, r = t / pairs;
+        const int ja = j0 + 2 * s, jb = ja + 1;
         wfc A = {0.0, 0.0}, B = {0.0, 0.0};
-        int conj = r >= hx;
-        int rr = conj ? nx - r : r;
+        const int conj = r >= hx;
+        const int rr = conj ? nx - r : r;
         if (ja < ny) {
-            double2 x = src[(long)ja * hx + rr];
+            double2 x = src[(long)rr * ldj + ja];
             A.re = x.x;
             A.im = x.y;
             if (jb < ny) {
-                double2 y = src[(long)jb * hx + rr];
+                double2 y = src[(long)rr * ldj + jb];
                 B.re = y.x;
                 B.im = y.y;
             }
@@ -395,50 +547,66 @@ __global__ __launch_bounds__(256) void k_inv_fx(WfConst c, WfSpectral sp, double
         bufa[s * nx + r] = wf_pack_pair(A, B, conj);
     }
     __syncthreads();
-    wfc *res;
-    if constexpr (NX > 0)
-        res = wf_fft_ct<NX>(bufa, bufb, pairs, tw);
-    else
-        res = wf_lds_fft(bufa, bufb, nx, pairs, rad, tw);
-    double scale = (double)nx * (double)ny;
+    wfc *res = wf_transform<NX, PAIRS>(bufa, bufb, nx, pairs, rad, tw);
+    const double scale = (double)nx * (double)ny;
     for (int t = threadIdx.x; t < pairs * Nx; t += blockDim.x) {
-        int s = t / Nx, i = t - s * Nx;
-        int ja = 2 * (jpair0 + s), jb = ja + 1;
+        const int s = t / Nx, i = t - s * Nx;
+        const int ja = j0 + 2 * s, jb = ja + 1;
         if (ja >= ny) continue;
-        int ii = (i == Nx - 1) ? 0 : i;
-        double pa = res[s * nx + ii].re / scale;
-        long qa = wf_q(c, i, ja, k);
-        double da = fabs(pa - p[qa]);
+        const int ii = (i == Nx - 1) ? 0 : i;
+        const double pa = res[s * nx + ii].re / scale;
+        const long qa = wf_q(c, i, ja, k);
+        const double da = fabs(pa - p[qa]);
         dmax = (da > dmax) ? da : dmax;
         p[qa] = pa;
         if (ja == 0) { /* duplicate row j = Ny-1 */
-            long qd = wf_q(c, i, c.Ny - 1, k);
-            double dd = fabs(pa - p[qd]);
+            const long qd = wf_q(c, i, c.Ny - 1, k);
+            const double dd = fabs(pa - p[qd]);
             dmax = (dd > dmax) ? dd : dmax;
             p[qd] = pa;
         }
         if (jb < ny) {
-            double pb = res[s * nx + ii].im / scale;
-            long qb = wf_q(c, i, jb, k);
-            double db = fabs(pb - p[qb]);
+            const double pb = res[s * nx + ii].im / scale;
+            const long qb = wf_q(c, i, jb, k);
+            const double db = fabs(pb - p[qb]);
             dmax = (db > dmax) ? db : dmax;
             p[qb] = pb;
         }
     }
-    wf_atomic_max_nonneg(&sp.ctl->norm_bits, dmax);
+    wf_block_max_store(dmax, &sp.pmax[blockIdx.x]);
 }
 
 /* ------------------------------------------------------------- launchers ---- */
 
-#define WF_DISPATCH_N(n, KERNEL, grd, blk, lds, s, ...)                                        \
-    do {                                                                                      \
-        switch (n) {                                                                          \
-            case 63: hipLaunchKernelGGL(KERNEL<63>, grd, blk, lds, s, __VA_ARGS__); break;    \
-            case 127: hipLaunchKernelGGL(KERNEL<127>, grd, blk, lds, s, __VA_ARGS__); break;  \
-            case 255: hipLaunchKernelGGL(KERNEL<255>, grd, blk, lds, s, __VA_ARGS__); break;  \
-            case 511: hipLaunchKernelGGL(KERNEL<511>, grd, blk, lds, s, __VA_ARGS__); break;  \
-            default: hipLaunchKernelGGL(KERNEL<0>, grd, blk, lds, s, __VA_ARGS__); break;     \
-        }                                                                                     \
+/* lengths with a compile-time plan */
+static bool wf_planned(int n) { return n == 63 || n == 127 || n == 255 || n == 511; }
+
+/* dynamic LDS of an x/y transform kernel: twiddles + one buffer (compile-time
+ * plan, in place) or two buffers (run-time radices) */
+static size_t wf_fft_lds(int n, int nseq) {
+    return sizeof(wfc) * (size_t)n * (1 + (wf_planned(n) ? 1 : 2) * (size_t)nseq);
+}
+
+#define WF_DISPATCH_N(n, KERNEL, grd, blk, lds, s, ...)                                                  \
+    do {                                                                                                \
+        switch (n) {                                                                                    \
+            case 63: hipLaunchKernelGGL(KERNEL<63>, grd, blk, lds, s, __VA_ARGS__); break;              \
+            case 127: hipLaunchKernelGGL(KERNEL<127>, grd, blk, lds, s, __VA_ARGS__); break;            \
+            case 255: hipLaunchKernelGGL(KERNEL<255>, grd, blk, lds, s, __VA_ARGS__); break;            \
+            case 511: hipLaunchKernelGGL(KERNEL<511>, grd, blk, lds, s, __VA_ARGS__); break;            \
+            default: hipLaunchKernelGGL(KERNEL<0>, grd, blk, lds, s, __VA_ARGS__); break;               \
+        }                                                                                               \
+    } while (0)
+
+#define WF_DISPATCH_NM(n, MODE, grd, blk, lds, s, ...)                                                   \
+    do {                                                                                                \
+        switch (n) {                                                                                    \
+            case 63: hipLaunchKernelGGL((k_rhs_fx<63, MODE>), grd, blk, lds, s, __VA_ARGS__); break;    \
+            case 127: hipLaunchKernelGGL((k_rhs_fx<127, MODE>), grd, blk, lds, s, __VA_ARGS__); break;  \
+            case 255: hipLaunchKernelGGL((k_rhs_fx<255, MODE>), grd, blk, lds, s, __VA_ARGS__); break;  \
+            case 511: hipLaunchKernelGGL((k_rhs_fx<511, MODE>), grd, blk, lds, s, __VA_ARGS__); break;  \
+            default: hipLaunchKernelGGL((k_rhs_fx<0, MODE>), grd, blk, lds, s, __VA_ARGS__); break;     \
+        }                                                                                               \
     } while (0)
 
 hipError_t wf_launch_density(const WfConst &c, const double *T, double *rho, hipStream_t s) {
@@ -451,54 +619,58 @@ hipError_t wf_launch_iter_reset(WfIterCtl *ctl, hipStream_t s) {
     return hipGetLastError();
 }
 
-hipError_t wf_launch_iter_check(WfIterCtl *ctl, double tol, int m, int max_iter, hipStream_t s) {
-    hipLaunchKernelGGL(k_iter_check, dim3(1), dim3(1), 0, s, ctl, tol, m, max_iter);
+hipError_t wf_launch_iter_check(const WfSpectral &sp, double tol, int m, int max_iter, hipStream_t s) {
+    hipLaunchKernelGGL(k_iter_check, dim3(1), dim3(WF_NT), 0, s, sp.ctl, (const double *)sp.pmax, sp.npmax, tol, m,
+                       max_iter);
     return hipGetLastError();
 }
 
 hipError_t wf_launch_pressure_rhs_fx(const WfConst &c, const WfSpectral &sp, const WfState &Ys, const WfState &Yn,
-                                     const double *p, int first_iter, hipStream_t s) {
-    int npairs = (sp.ny + 1) / 2;
-    dim3 grd((npairs + sp.fx_pairs - 1) / sp.fx_pairs, sp.nz - 1, 1);
-    size_t lds = sizeof(wfc) * (size_t)sp.nx * (1 + 2 * sp.fx_pairs);
-    WF_DISPATCH_N(sp.nx, k_rhs_fx, grd, dim3(256), lds, s, c, sp, Ys, Yn, p, first_iter, 0);
+                                     const double *p, int mode, hipStream_t s) {
+    const int nslabs = (mode == WF_F_TOP) ? 1 : sp.nz - 1;
+    dim3 grd((unsigned)(sp.ngroups * nslabs));
+    size_t lds = wf_fft_lds(sp.nx, sp.fx_pairs);
+    switch (mode) {
+        case WF_F_ITER: WF_DISPATCH_NM(sp.nx, WF_F_ITER, grd, dim3(WF_NT), lds, s, c, sp, Ys, Yn, p); break;
+        case WF_F_FIRST: WF_DISPATCH_NM(sp.nx, WF_F_FIRST, grd, dim3(WF_NT), lds, s, c, sp, Ys, Yn, p); break;
+        case WF_F_CONST: WF_DISPATCH_NM(sp.nx, WF_F_CONST, grd, dim3(WF_NT), lds, s, c, sp, Ys, Yn, p); break;
+        default: WF_DISPATCH_NM(sp.nx, WF_F_TOP, grd, dim3(WF_NT), lds, s, c, sp, Ys, Yn, p); break;
+    }
     return hipGetLastError();
 }
 
-hipError_t wf_launch_ptop_fx(const WfConst &c, const WfSpectral &sp, const double *p, hipStream_t s) {
-    int npairs = (sp.ny + 1) / 2;
-    dim3 grd((npairs + sp.fx_pairs - 1) / sp.fx_pairs, 1, 1);
-    WfState none = {0, 0, 0, 0, 0};
-    size_t lds = sizeof(wfc) * (size_t)sp.nx * (1 + 2 * sp.fx_pairs);
-    WF_DISPATCH_N(sp.nx, k_rhs_fx, grd, dim3(256), lds, s, c, sp, none, none, p, 0, 1);
-    return hipGetLastError();
-}
-
-hipError_t wf_launch_fy(const WfSpectral &sp, double2 *buf, int slab0, int nslabs, int sign, int guarded,
-                        hipStream_t s) {
+hipError_t wf_launch_fy(const WfSpectral &sp, double2 *buf, long slab_stride, int slab0, int nslabs, int sign,
+                        int guarded, hipStream_t s) {
     if (nslabs <= 0) return hipSuccess;
-    dim3 grd((sp.hx + sp.fy_cols - 1) / sp.fy_cols, nslabs, 1);
-    size_t lds = sizeof(wfc) * (size_t)sp.ny * (1 + 2 * sp.fy_cols);
-    WF_DISPATCH_N(sp.ny, k_fy, grd, dim3(256), lds, s, sp, buf, slab0, sign, guarded);
+    dim3 grd((sp.hx + WF_FY_COLS - 1) / WF_FY_COLS, nslabs, 1);
+    size_t lds = wf_fft_lds(sp.ny, WF_FY_COLS);
+    WF_DISPATCH_N(sp.ny, k_fy, grd, dim3(WF_NT), lds, s, sp, buf, slab_stride, slab0, sign, guarded);
     return hipGetLastError();
 }
 
-hipError_t wf_launch_pivots(const WfSpectral &sp, double *U, hipStream_t s) {
-    dim3 grd((sp.hx + 63) / 64, sp.ny, 1);
-    hipLaunchKernelGGL(k_pivots, grd, dim3(64), 0, s, sp, U);
+hipError_t wf_launch_pivots(const WfSpectral &sp, double *L, double *RU, hipStream_t s) {
+    dim3 grd((sp.ny + 63) / 64, sp.hx, 1);
+    hipLaunchKernelGGL(k_pivots, grd, dim3(64), 0, s, sp, L, RU);
     return hipGetLastError();
 }
 
 hipError_t wf_launch_thomas(const WfSpectral &sp, hipStream_t s) {
-    dim3 grd((sp.hx + 63) / 64, sp.ny, 1);
-    hipLaunchKernelGGL(k_thomas, grd, dim3(64), 0, s, sp);
+    dim3 grd((sp.ny + 63) / 64, sp.hx, 1);
+    if (sp.nz <= 8 * WF_THOMAS_MAX_CHUNKS) {
+        const int chunks = (sp.nz + 7) / 8;
+        hipLaunchKernelGGL(k_thomas<8>, grd, dim3(64 * chunks), 0, s, sp);
+    } else if (sp.nz <= 16 * WF_THOMAS_MAX_CHUNKS) {
+        const int chunks = (sp.nz + 15) / 16;
+        hipLaunchKernelGGL(k_thomas<16>, grd, dim3(64 * chunks), 0, s, sp);
+    } else {
+        return hipErrorInvalidValue;
+    }
     return hipGetLastError();
 }
 
 hipError_t wf_launch_inverse_fx(const WfConst &c, const WfSpectral &sp, double *p, hipStream_t s) {
-    int npairs = (sp.ny + 1) / 2;
-    dim3 grd((npairs + sp.fx_pairs - 1) / sp.fx_pairs, sp.nz + 1, 1);
-    size_t lds = sizeof(wfc) * (size_t)sp.nx * (1 + 2 * sp.fx_pairs);
-    WF_DISPATCH_N(sp.nx, k_inv_fx, grd, dim3(256), lds, s, c, sp, p);
+    dim3 grd((unsigned)sp.npmax);
+    size_t lds = wf_fft_lds(sp.nx, sp.fx_pairs);
+    WF_DISPATCH_N(sp.nx, k_inv_fx, grd, dim3(WF_NT), lds, s, c, sp, p);
     return hipGetLastError();
 }

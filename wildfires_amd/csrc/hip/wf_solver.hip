@@ -55,7 +55,7 @@ struct wf_solver {
     double *state[4];
     int i_yn, i_x0, i_x1, i_acc;
     double *derived;
-    double *p, *A, *U;
+    double *p, *A, *L, *RU, *pmax;
     double2 *F, *Ptop;
     double *z, *cd2, *topo, *kx2, *ky2, *ta, *tc, *beta, *x, *y;
     int *cut, *nzy, *radx, *rady;
@@ -172,6 +172,7 @@ static int wf_validate(const Parameters *P) {
     if (P->Nx - 1 > 1024 || P->Ny - 1 > 1024)
         return wf_fail(WF_E_ARG, "Nx-1 and Ny-1 must not exceed 1024 (got %d, %d)", P->Nx - 1, P->Ny - 1);
     if (P->Nz < 6) return wf_fail(WF_E_ARG, "Nz must be at least 6 (got %d)", P->Nz);
+    if (P->Nz - 1 > WF_NZ_MAX) return wf_fail(WF_E_ARG, "Nz-1 must not exceed %d (got %d)", WF_NZ_MAX, P->Nz - 1);
     if (P->z == NULL || P->cut_nodes == NULL || P->Nz_Y == NULL || P->topography == NULL)
         return wf_fail(WF_E_ARG, "parameters have no grids or terrain mask");
     long cols = (long)P->Nx * P->Ny;
@@ -303,8 +304,11 @@ static int wf_create_impl(wf_solver *s, const Parameters *P) {
     sp.ny = Ny - 1;
     sp.nz = Nz - 1;
     sp.hx = sp.nx / 2 + 1;
-    sp.fx_pairs = (sp.nx <= 300) ? 4 : 2;
-    sp.fy_cols = (sp.ny <= 300) ? 4 : 2;
+    sp.ldj = (sp.ny + 7) & ~7;
+    sp.fx_pairs = wf_pairs_for(sp.nx);
+    sp.fy_cols = WF_FY_COLS;
+    sp.ngroups = ((sp.ny + 1) / 2 + sp.fx_pairs - 1) / sp.fx_pairs;
+    sp.npmax = sp.ngroups * sp.nz + 1;
     int radx[WF_MAX_RADICES], rady[WF_MAX_RADICES];
     if (wf_factor(sp.nx, radx) < 0 || wf_factor(sp.ny, rady) < 0)
         return wf_fail(WF_E_ARG, "transform length has too many factors");
@@ -347,16 +351,21 @@ static int wf_create_impl(wf_solver *s, const Parameters *P) {
     sp.b0 = -1.0 / dz0;
     sp.beta_top = 2 / (dzNm2 * dzNm1);
     sp.top_div = 2 / (dzNm1 * (dzNm2 + dzNm1));
-    long nspec = (long)sp.nz * sp.ny * sp.hx;
+    long nspec = (long)sp.nz * sp.hx * sp.ldj;
     WF_HIP(hipMalloc((void **)&s->F, sizeof(double2) * nspec));
     WF_HIP(hipMemset(s->F, 0, sizeof(double2) * nspec));
-    WF_HIP(hipMalloc((void **)&s->Ptop, sizeof(double2) * (long)sp.ny * sp.hx));
-    WF_HIP(hipMemset(s->Ptop, 0, sizeof(double2) * (long)sp.ny * sp.hx));
-    WF_HIP(hipMalloc((void **)&s->U, sizeof(double) * nspec));
+    WF_HIP(hipMalloc((void **)&s->Ptop, sizeof(double2) * (long)sp.hx * sp.ldj));
+    WF_HIP(hipMemset(s->Ptop, 0, sizeof(double2) * (long)sp.hx * sp.ldj));
+    WF_HIP(hipMalloc((void **)&s->L, sizeof(double) * nspec));
+    WF_HIP(hipMalloc((void **)&s->RU, sizeof(double) * nspec));
+    WF_HIP(hipMalloc((void **)&s->pmax, sizeof(double) * sp.npmax));
+    WF_HIP(hipMemset(s->pmax, 0, sizeof(double) * sp.npmax));
     sp.F = s->F;
     sp.Ptop = s->Ptop;
     sp.A = s->A;
-    sp.U = s->U;
+    sp.L = s->L;
+    sp.RU = s->RU;
+    sp.pmax = s->pmax;
     sp.kx2 = s->kx2;
     sp.ky2 = s->ky2;
     sp.ta = s->ta;
@@ -368,7 +377,7 @@ static int wf_create_impl(wf_solver *s, const Parameters *P) {
     sp.rady = s->rady;
     sp.rho = s->rho;
     sp.ctl = s->ctl;
-    WF_HIP(wf_launch_pivots(sp, s->U, s->stream));
+    WF_HIP(wf_launch_pivots(sp, s->L, s->RU, s->stream));
     WF_HIP(hipStreamSynchronize(s->stream));
     for (int q = 0; q < 2 * WF_MAX_EVENTS; q++) WF_HIP(hipEventCreate(&s->ev[q]));
     return WF_OK;
@@ -395,7 +404,7 @@ extern "C" int wf_solver_destroy(wf_solver *s) {
     if (s == NULL) return WF_OK;
     if (s->stream) (void)hipStreamSynchronize(s->stream);
     for (int q = 0; q < 4; q++) (void)hipFree(s->state[q]);
-    void *bufs[] = {s->derived, s->p, s->A, s->U, s->F, s->Ptop, s->z, s->cd2, s->topo, s->kx2, s->ky2, s->ta,
+    void *bufs[] = {s->derived, s->p, s->A, s->L, s->RU, s->pmax, s->F, s->Ptop, s->z, s->cd2, s->topo, s->kx2, s->ky2, s->ta,
                     s->tc, s->beta, s->x, s->y, s->cut, s->nzy, s->radx, s->rady, s->twx, s->twy, s->scratch,
                     s->Tsrc, s->rho, s->ctl};
     for (size_t q = 0; q < sizeof(bufs) / sizeof(bufs[0]); q++)
@@ -447,8 +456,9 @@ static int wf_get_field(wf_solver *s, const double *src, double *f_host) {
 /* spectrum of the top pressure plane (fixed for the whole run: the solve never
  * writes the unique nodes of k = Nz-1) */
 static int wf_top_spectrum(wf_solver *s) {
-    WF_HIP(wf_launch_ptop_fx(s->c, s->sp, s->p, s->stream));
-    WF_HIP(wf_launch_fy(s->sp, s->Ptop, 0, 1, -1, 0, s->stream));
+    WfState none = {0, 0, 0, 0, 0};
+    WF_HIP(wf_launch_pressure_rhs_fx(s->c, s->sp, none, none, s->p, WF_F_TOP, s->stream));
+    WF_HIP(wf_launch_fy(s->sp, s->Ptop, 0, 0, 1, -1, 0, s->stream));
     return WF_OK;
 }
 
@@ -525,12 +535,14 @@ static int wf_pressure_solve(wf_solver *s, double *Ys_buf, double *Yn_buf) {
     }
     if (P.variable_density) WF_LAUNCH(s, WF_K_OTHER, wf_launch_density(s->c, yn.T, s->rho, s->stream));
     for (int m = 0; m < max_iter; m++) {
-        WF_LAUNCH(s, WF_K_PRHS, wf_launch_pressure_rhs_fx(s->c, s->sp, ys, yn, s->p, m == 0, s->stream));
-        WF_LAUNCH(s, WF_K_PFY, wf_launch_fy(s->sp, s->F, 1, s->sp.nz - 1, -1, 1, s->stream));
+        const int form = (m > 0) ? WF_F_ITER : (P.variable_density ? WF_F_FIRST : WF_F_CONST);
+        const long sstride = (long)s->sp.hx * s->sp.ldj;
+        WF_LAUNCH(s, WF_K_PRHS, wf_launch_pressure_rhs_fx(s->c, s->sp, ys, yn, s->p, form, s->stream));
+        WF_LAUNCH(s, WF_K_PFY, wf_launch_fy(s->sp, s->F, sstride, 1, s->sp.nz - 1, -1, 1, s->stream));
         WF_LAUNCH(s, WF_K_THOMAS, wf_launch_thomas(s->sp, s->stream));
-        WF_LAUNCH(s, WF_K_PFY, wf_launch_fy(s->sp, s->F, 0, s->sp.nz, +1, 1, s->stream));
+        WF_LAUNCH(s, WF_K_PFY, wf_launch_fy(s->sp, s->F, sstride, 0, s->sp.nz, +1, 1, s->stream));
         WF_LAUNCH(s, WF_K_PINV, wf_launch_inverse_fx(s->c, s->sp, s->p, s->stream));
-        WF_LAUNCH(s, WF_K_OTHER, wf_launch_iter_check(s->ctl, P.pressure_solver_tol, m, max_iter, s->stream));
+        WF_LAUNCH(s, WF_K_OTHER, wf_launch_iter_check(s->sp, P.pressure_solver_tol, m, max_iter, s->stream));
         if ((m + 1) % WF_POLL == 0 && m + 1 < max_iter) {
             int done = 0;
             WF_HIP(hipMemcpyAsync(&done, &s->ctl->done, sizeof(int), hipMemcpyDeviceToHost, s->stream));
