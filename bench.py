@@ -145,7 +145,7 @@ def run(args):
                                "layer 0.5 m, line ignition, RK4, dt=0.005 s, variable density",
                    "cells": N, "pressure_solves_per_step": round(solves, 3), "state_finite": finite},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(dom),
                      "algorithmic_bytes_per_launch": int(kbytes[dom]),
                      "mean_launch_us": round(dom_mean_s * 1e6, 2)},
         "step_roofline": {"bytes_per_cell_step": round(b_step, 1), "achieved": round(step_achieved, 1),
@@ -238,6 +238,98 @@ def rocprof_summary(directory):
     return "\n".join(lines)
 
 
+KERNEL_CLASSES = (("k_derived", "derived"), ("k_stage", "stage"), ("k_rhs_fx", "pressure_rhs_fx"),
+                  ("k_fy", "pressure_fy"), ("k_thomas", "thomas"), ("k_inv_fx", "pressure_inverse_fx"),
+                  ("k_project", "project"))
+
+
+def _kernel_class(name):
+    """bench kernel class of a rocprofv3 kernel name (None if not timed by
+    class); the top-plane transform k_rhs_fx<n, 3> runs once per run, not per
+    solve, and is left out."""
+    short = _short_name(name)
+    if short.startswith("k_rhs_fx") and short.replace(" ", "").endswith(",3>"):
+        return None
+    base = short.split("<")[0]
+    for prefix, cls in KERNEL_CLASSES:
+        if base == prefix or (prefix == "k_stage" and base == "k_stage_columns"):
+            return cls
+    return None
+
+
+def source_digest():
+    """sha256 over the HIP sources: ties a traffic measurement to the kernels
+    it was taken on."""
+    import glob
+    import hashlib
+    h = hashlib.sha256()
+    for path in sorted(glob.glob(os.path.join(ROOT, "wildfires_amd", "csrc", "hip", "*"))):
+        with open(path, "rb") as f:
+            h.update(os.path.basename(path).encode())
+            h.update(f.read())
+    return h.hexdigest()
+
+
+def _pmc_by_dispatch(directory):
+    import csv
+    import glob
+    out = {}
+    for path in glob.glob(os.path.join(directory, "**", "*counter_collection.csv"), recursive=True):
+        with open(path) as f:
+            for r in csv.DictReader(f):
+                key = r.get("Dispatch_Id", r.get("Correlation_Id"))
+                out[key] = (r["Kernel_Name"], float(r["Counter_Value"]))
+    return out
+
+
+def traffic_json(pmc_dir, cal_dir, copy_bytes):
+    """Measured HBM bytes per launch of each kernel class from rocprofv3 --pmc
+    FETCH_SIZE (pmc_dir/fetch) and WRITE_SIZE (pmc_dir/write) runs, each scaled
+    by what the counter reads for a device-to-device copy of copy_bytes
+    (cal_dir/cal_fetch, cal_dir/cal_write)."""
+    scales = {}
+    for counter, sub in (("FETCH_SIZE", "cal_fetch"), ("WRITE_SIZE", "cal_write")):
+        vals = [v for (n, v) in _pmc_by_dispatch(os.path.join(cal_dir, sub)).values() if "copyBuffer" in n]
+        if not vals:
+            raise SystemExit("no copy dispatches under %s" % os.path.join(cal_dir, sub))
+        measured = sum(vals) / len(vals) * 1024.0
+        scales[counter] = copy_bytes / measured
+    acc = {}
+    for counter, sub in (("FETCH_SIZE", "fetch"), ("WRITE_SIZE", "write")):
+        for name, v in _pmc_by_dispatch(os.path.join(pmc_dir, sub)).values():
+            cls = _kernel_class(name)
+            if cls is None:
+                continue
+            a = acc.setdefault(cls, {"FETCH_SIZE": [0.0, 0], "WRITE_SIZE": [0.0, 0]})
+            # k_stage_columns is a companion launch of k_stage: add, do not count
+            a[counter][0] += v * 1024.0 * scales[counter]
+            if not _short_name(name).startswith("k_stage_columns"):
+                a[counter][1] += 1
+    kernels = {}
+    for cls, a in sorted(acc.items()):
+        fb = a["FETCH_SIZE"][0] / max(a["FETCH_SIZE"][1], 1)
+        wb = a["WRITE_SIZE"][0] / max(a["WRITE_SIZE"][1], 1)
+        kernels[cls] = {"bytes_per_launch": round(fb + wb), "fetch_bytes": round(fb), "write_bytes": round(wb),
+                        "launches": a["FETCH_SIZE"][1]}
+    return {"source_sha256": source_digest(), "copy_bytes": copy_bytes,
+            "scale": {k: round(v, 4) for k, v in scales.items()}, "kernels": kernels}
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed counter summary, if
+    it was measured on the current sources; else None."""
+    path = os.path.join(ROOT, "profiles", "traffic_hill256.json")
+    try:
+        with open(path) as f:
+            t = json.load(f)
+    except (OSError, ValueError):
+        return None
+    if t.get("source_sha256") != source_digest():
+        return None
+    k = t.get("kernels", {}).get(kernel)
+    return None if k is None else k["bytes_per_launch"]
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -250,9 +342,16 @@ def main():
                     help="only run device-to-device copies of BYTES (counter scale factor under rocprofv3 --pmc)")
     ap.add_argument("--rocprof-summary", metavar="DIR",
                     help="print a markdown summary of rocprofv3 CSV output under DIR and exit")
+    ap.add_argument("--traffic-json", nargs=2, metavar=("PMC_DIR", "CAL_DIR"),
+                    help="print per-kernel HBM bytes per launch from FETCH_SIZE/WRITE_SIZE runs under "
+                         "PMC_DIR/{fetch,write} scaled by the copy runs under CAL_DIR/{cal_fetch,cal_write}")
     args = ap.parse_args()
     if args.rocprof_summary:
         print(rocprof_summary(args.rocprof_summary))
+        return
+    if args.traffic_json:
+        print(json.dumps(traffic_json(args.traffic_json[0], args.traffic_json[1], args.copy_calibration or 1 << 30),
+                         indent=1))
         return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (every box has one GPU)",

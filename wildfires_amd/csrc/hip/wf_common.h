@@ -49,6 +49,7 @@ struct WfState {
 struct WfDerived {
     double *mu, *s11, *s12, *s13, *s22, *s23, *s33, *divU;
     double *utau; /* plane */
+    double *col;  /* 4 x 5 planes: raw RHS (u,v,w,T,Y) at cut+1, cut+2, Nz-2, Nz-3 of each column */
 };
 
 __device__ __forceinline__ long wf_q(const WfConst &c, int i, int j, int k) {
@@ -107,9 +108,36 @@ __device__ __forceinline__ double wf_fw(double zibm, double utau, double nu) {
 }
 
 /* Distance to the terrain max(z_k - topo, 0) (src/cu/ibm.cu:84). */
-__device__ __forceinline__ double wf_zibm(const WfConst &c, int col, int k) {
-    double d = c.z[k] - c.topo[col];
+__device__ __forceinline__ double wf_zibm_t(const WfConst &c, double topo, int k) {
+    double d = c.z[k] - topo;
     return (d > 0.0) ? d : 0.0;
+}
+__device__ __forceinline__ double wf_zibm(const WfConst &c, int col, int k) { return wf_zibm_t(c, c.topo[col], k); }
+
+/* Workgroups are dispatched round-robin over the 8 XCDs (each with its own
+ * L2); this maps the hardware block index to a logical tile so that each XCD
+ * works on one contiguous range of tiles and neighbouring tiles share an L2. */
+__device__ __forceinline__ int wf_xcd_tile(int b, int nblocks) {
+    const int per = nblocks >> 3, rem = nblocks & 7;
+    const int x = b & 7, idx = b >> 3;
+    return x * per + (x < rem ? x : rem) + idx;
+}
+
+/* Node tiles of the stencil passes: WF_TILE_X x WF_TILE_Y columns, one plane.
+ * Logical tile order: k fastest, then y, then x, so consecutive tiles of an
+ * XCD are the z-neighbours (then y-neighbours) whose stencils overlap. */
+#define WF_TILE_X 64
+#define WF_TILE_Y 4
+__host__ __device__ __forceinline__ int wf_tile_count(const WfConst &c) {
+    return ((c.Nx + WF_TILE_X - 1) / WF_TILE_X) * ((c.Ny + WF_TILE_Y - 1) / WF_TILE_Y) * c.Nz;
+}
+__device__ __forceinline__ void wf_tile_ijk(const WfConst &c, int *i, int *j, int *k) {
+    const int t = wf_xcd_tile(blockIdx.x, gridDim.x);
+    const int nty = (c.Ny + WF_TILE_Y - 1) / WF_TILE_Y;
+    *k = t % c.Nz;
+    const int col = t / c.Nz;
+    *j = (col % nty) * WF_TILE_Y + threadIdx.y;
+    *i = (col / nty) * WF_TILE_X + threadIdx.x;
 }
 
 #define WF_CHECK_LAUNCH() (hipGetLastError())

@@ -61,15 +61,6 @@ __device__ __forceinline__ wfc *wf_transform(wfc *buf, wfc *bufb, int n, int nse
     }
 }
 
-/* Workgroups are dispatched round-robin over the 8 XCDs (each with its own
- * L2); this maps the hardware block index to a logical tile so that each XCD
- * works on one contiguous range of tiles and neighbouring tiles share an L2. */
-__device__ __forceinline__ int wf_xcd_tile(int b, int nblocks) {
-    const int per = nblocks >> 3, rem = nblocks & 7;
-    const int x = b & 7, idx = b >> 3;
-    return x * per + (x < rem ? x : rem) + idx;
-}
-
 /* Maximum of v over the workgroup, written by thread 0 to *out. */
 __device__ __forceinline__ void wf_block_max_store(double v, double *out) {
     __shared__ double red[WF_NT / 64];
@@ -227,9 +218,12 @@ __global__ __launch_bounds__(WF_NT) void k_rhs_fx(WfConst c, WfSpectral sp, WfSt
     const int pairs = (NX > 0) ? PAIRS : sp.fx_pairs;
     const int ny = sp.ny, hx = sp.hx, ldj = sp.ldj;
     wfc *tw = smem, *bufa = smem + nx, *bufb = bufa + pairs * nx;
+    /* logical tile order: slab fastest, so an XCD's consecutive tiles are the
+     * z-neighbours whose stencils overlap */
     const int tile = wf_xcd_tile(blockIdx.x, gridDim.x);
-    const int g = tile % sp.ngroups;
-    const int k = (MODE == WF_F_TOP) ? c.Nz - 1 : 1 + tile / sp.ngroups;
+    const int nsl = (MODE == WF_F_TOP) ? 1 : sp.nz - 1;
+    const int g = tile / nsl;
+    const int k = (MODE == WF_F_TOP) ? c.Nz - 1 : 1 + tile % nsl;
     const int j0 = 2 * g * pairs;
     for (int t = threadIdx.x; t < nx; t += blockDim.x) tw[t] = *(const wfc *)&sp.twx[t];
     if (NX == 0 && threadIdx.x < WF_MAX_RADICES) rad[threadIdx.x] = sp.radx[threadIdx.x];

@@ -99,6 +99,7 @@ static WfDerived wf_derived_view(const wf_solver *s) {
     d.s33 = b + 6 * N;
     d.divU = b + 7 * N;
     d.utau = b + 8 * N;
+    d.col = b + 8 * N + s->plane;
     return d;
 }
 
@@ -173,6 +174,8 @@ static int wf_validate(const Parameters *P) {
         return wf_fail(WF_E_ARG, "Nx-1 and Ny-1 must not exceed 1024 (got %d, %d)", P->Nx - 1, P->Ny - 1);
     if (P->Nz < 6) return wf_fail(WF_E_ARG, "Nz must be at least 6 (got %d)", P->Nz);
     if (P->Nz - 1 > WF_NZ_MAX) return wf_fail(WF_E_ARG, "Nz-1 must not exceed %d (got %d)", WF_NZ_MAX, P->Nz - 1);
+    if ((long)P->Nx * P->Ny * P->Nz > (1L << 28))
+        return wf_fail(WF_E_ARG, "Nx*Ny*Nz must not exceed 2^28 (got %ld)", (long)P->Nx * P->Ny * P->Nz);
     if (P->z == NULL || P->cut_nodes == NULL || P->Nz_Y == NULL || P->topography == NULL)
         return wf_fail(WF_E_ARG, "parameters have no grids or terrain mask");
     long cols = (long)P->Nx * P->Ny;
@@ -211,8 +214,8 @@ static int wf_create_impl(wf_solver *s, const Parameters *P) {
     s->i_x0 = 1;
     s->i_x1 = 2;
     s->i_acc = 3;
-    WF_HIP(hipMalloc((void **)&s->derived, sizeof(double) * (8 * s->N + s->plane)));
-    WF_HIP(hipMemset(s->derived, 0, sizeof(double) * (8 * s->N + s->plane)));
+    WF_HIP(hipMalloc((void **)&s->derived, sizeof(double) * (8 * s->N + 21 * s->plane)));
+    WF_HIP(hipMemset(s->derived, 0, sizeof(double) * (8 * s->N + 21 * s->plane)));
     WF_HIP(hipMalloc((void **)&s->p, sizeof(double) * s->N));
     WF_HIP(hipMemset(s->p, 0, sizeof(double) * s->N));
     WF_HIP(hipMalloc((void **)&s->A, sizeof(double) * s->N));
