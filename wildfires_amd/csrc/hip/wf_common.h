@@ -7,9 +7,9 @@
  * The reference layout (z fastest) exists only at the C-ABI boundary and is
  * converted by the layout kernels.
  *
- * All FP64 kernels are compiled without FMA contraction so that every
- * expression is evaluated in the operation order written, which is the order
- * of the reference's expressions (src/cu/pde.cu, turbulence.cu, pressure.cu).
+ * FP64 throughout.  Kernels are compiled without FMA contraction, so an
+ * expression evaluates in the order written; the stencil passes and the FFT
+ * opt in to contraction and take constant divisors as reciprocals where noted.
  */
 #ifndef WF_COMMON_H
 #define WF_COMMON_H
@@ -32,12 +32,33 @@ struct WfConst {
     double u_dead, v_dead, w_dead, T_dead, Y_dead;
     int temperature_convection, suth_mu, suth_k, variable_density, fd_z;
     int exchange_on; /* alpha_s * sigma_s != 0: the h_c exchange term is evaluated */
+    /* reciprocals of the constant divisors of the stencil passes */
+    double inv2dx, inv2dy, invdx2, invdy2; /* 1/(2dx), 1/(2dy), 1/dx^2, 1/dy^2 */
+    double inv_cp, inv_Trho, inv_mu, inv_Pr; /* 1/c_p, 1/(T_inf rho_inf), 1/mu, 1/Pr */
+    const double *zc;      /* WF_ZC z-stencil coefficients per plane (WF_Z_*) */
     const double *z;       /* Nz */
     const double *cd2;     /* Nz: (C_s (dx dy dz_k)^(1/3))^2 evaluated on the host */
     const double *topo;    /* plane */
     const int *cut;        /* plane */
     const int *nzy;        /* plane */
 };
+
+/* Coefficients of the non-uniform z-stencils of plane k (src/cu/pde.cu:363-385,
+ * 497-500, 539-542, 572), stored at zc[WF_ZC k + WF_Z_*]:
+ *   D1F  one-sided first derivative from k, k+1, k+2       (k <= Nz-3)
+ *   D1B  one-sided first derivative from k-2, k-1, k       (k >= 2)
+ *   D2M  interior second derivative from k-1, k, k+1       (1 <= k <= Nz-2)
+ *   DZC  1/(z[k+1] - z[k-1]) of the central first derivative
+ *   D2F  four-point second derivative from k..k+3          (k = 0)
+ *   D2B  four-point second derivative from k-3..k          (k = Nz-1)
+ * Entries outside their range are 0. */
+#define WF_ZC 18
+#define WF_Z_D1F 0
+#define WF_Z_D1B 3
+#define WF_Z_D2M 6
+#define WF_Z_DZC 9
+#define WF_Z_D2F 10
+#define WF_Z_D2B 14
 
 /* Device view of a state vector [u|v|w|T|Y] in the internal layout. */
 struct WfState {
@@ -106,6 +127,11 @@ __device__ __forceinline__ double wf_fw(double zibm, double utau, double nu) {
     double a = -zibm * utau / WF_A_VD_D / nu;
     return (a < -38.0) ? 1.0 : 1 - exp(a);
 }
+/* The same with the reciprocal of nu. */
+__device__ __forceinline__ double wf_fw_inv(double zibm, double utau, double inv_nu) {
+    double a = -zibm * utau * (1.0 / WF_A_VD_D) * inv_nu;
+    return (a < -38.0) ? 1.0 : 1 - exp(a);
+}
 
 /* Distance to the terrain max(z_k - topo, 0) (src/cu/ibm.cu:84). */
 __device__ __forceinline__ double wf_zibm_t(const WfConst &c, double topo, int k) {
@@ -123,19 +149,54 @@ __device__ __forceinline__ int wf_xcd_tile(int b, int nblocks) {
     return x * per + (x < rem ? x : rem) + idx;
 }
 
-/* Node tiles of the stencil passes: WF_TILE_X x WF_TILE_Y columns, one plane.
- * Logical tile order: k fastest, then y, then x, so consecutive tiles of an
- * XCD are the z-neighbours (then y-neighbours) whose stencils overlap. */
+/* Node tiles of the stencil passes: WF_TILE_X x WF_TILE_Y columns, one plane. */
 #define WF_TILE_X 64
 #define WF_TILE_Y 4
+
+/* Order in which a launch of ncol column tiles x nk planes walks its tiles:
+ *   0  hardware order, column fastest;
+ *   1  each XCD a contiguous range, plane fastest (z-neighbours share an L2);
+ *   2  each XCD a contiguous range, column fastest;
+ *   3  each XCD a contiguous range, planes in groups of 8, then columns. */
+#ifndef WF_TILE_ORDER
+#define WF_TILE_ORDER 0
+#endif
+__device__ __forceinline__ void wf_tile_decode(int ncol, int nk, int *col, int *k) {
+#if WF_TILE_ORDER == 0
+    const int t = blockIdx.x;
+    *col = t % ncol;
+    *k = t / ncol;
+#elif WF_TILE_ORDER == 2
+    const int t = wf_xcd_tile(blockIdx.x, gridDim.x);
+    *col = t % ncol;
+    *k = t / ncol;
+#elif WF_TILE_ORDER == 3
+    const int t = wf_xcd_tile(blockIdx.x, gridDim.x);
+    const int nfull = nk >> 3;
+    if (t < nfull * 8 * ncol) {
+        const int g = t / (8 * ncol), r = t - g * 8 * ncol;
+        *col = r >> 3;
+        *k = g * 8 + (r & 7);
+    } else {
+        const int r = t - nfull * 8 * ncol, gs = nk - nfull * 8;
+        *col = r / gs;
+        *k = nfull * 8 + r % gs;
+    }
+#else
+    const int t = wf_xcd_tile(blockIdx.x, gridDim.x);
+    *k = t % nk;
+    *col = t / nk;
+#endif
+}
+
 __host__ __device__ __forceinline__ int wf_tile_count(const WfConst &c) {
     return ((c.Nx + WF_TILE_X - 1) / WF_TILE_X) * ((c.Ny + WF_TILE_Y - 1) / WF_TILE_Y) * c.Nz;
 }
 __device__ __forceinline__ void wf_tile_ijk(const WfConst &c, int *i, int *j, int *k) {
-    const int t = wf_xcd_tile(blockIdx.x, gridDim.x);
     const int nty = (c.Ny + WF_TILE_Y - 1) / WF_TILE_Y;
-    *k = t % c.Nz;
-    const int col = t / c.Nz;
+    const int ntx = (c.Nx + WF_TILE_X - 1) / WF_TILE_X;
+    int col;
+    wf_tile_decode(ntx * nty, c.Nz, &col, k);
     *j = (col % nty) * WF_TILE_Y + threadIdx.y;
     *i = (col / nty) * WF_TILE_X + threadIdx.x;
 }

@@ -218,12 +218,9 @@ __global__ __launch_bounds__(WF_NT) void k_rhs_fx(WfConst c, WfSpectral sp, WfSt
     const int pairs = (NX > 0) ? PAIRS : sp.fx_pairs;
     const int ny = sp.ny, hx = sp.hx, ldj = sp.ldj;
     wfc *tw = smem, *bufa = smem + nx, *bufb = bufa + pairs * nx;
-    /* logical tile order: slab fastest, so an XCD's consecutive tiles are the
-     * z-neighbours whose stencils overlap */
-    const int tile = wf_xcd_tile(blockIdx.x, gridDim.x);
-    const int nsl = (MODE == WF_F_TOP) ? 1 : sp.nz - 1;
-    const int g = tile / nsl;
-    const int k = (MODE == WF_F_TOP) ? c.Nz - 1 : 1 + tile % nsl;
+    int g, ks;
+    wf_tile_decode(sp.ngroups, (MODE == WF_F_TOP) ? 1 : sp.nz - 1, &g, &ks);
+    const int k = (MODE == WF_F_TOP) ? c.Nz - 1 : 1 + ks;
     const int j0 = 2 * g * pairs;
     for (int t = threadIdx.x; t < nx; t += blockDim.x) tw[t] = *(const wfc *)&sp.twx[t];
     if (NX == 0 && threadIdx.x < WF_MAX_RADICES) rad[threadIdx.x] = sp.radx[threadIdx.x];

@@ -57,7 +57,7 @@ struct wf_solver {
     double *derived;
     double *p, *A, *L, *RU, *pmax;
     double2 *F, *Ptop;
-    double *z, *cd2, *topo, *kx2, *ky2, *ta, *tc, *beta, *x, *y;
+    double *z, *cd2, *zc, *topo, *kx2, *ky2, *ta, *tc, *beta, *x, *y;
     int *cut, *nzy, *radx, *rady;
     double2 *twx, *twy;
     double *scratch, *Tsrc, *rho;
@@ -232,8 +232,48 @@ static int wf_create_impl(wf_solver *s, const Parameters *P) {
         double dz_k = (k < Nz - 1) ? z[k + 1] - z[k] : z[k] - z[k - 1];
         cd2[k] = pow(P->C_s * pow(P->dx * P->dy * dz_k, 1.0 / 3.0), 2.0);
     }
+    /* z-stencil coefficients: the weights of the reference's difference
+     * formulas (src/cu/pde.cu:363-385, 497-500, 539-542, 572) per plane */
+    std::vector<double> zc((size_t)WF_ZC * Nz, 0.0);
+    for (int k = 0; k < Nz; k++) {
+        double *e = &zc[(size_t)WF_ZC * k];
+        if (k <= Nz - 3) {
+            double h0 = z[k + 1] - z[k], h1 = z[k + 2] - z[k + 1];
+            e[WF_Z_D1F + 0] = -(2 * h0 + h1) / (h0 * (h0 + h1));
+            e[WF_Z_D1F + 1] = (h0 + h1) / (h0 * h1);
+            e[WF_Z_D1F + 2] = -h0 / (h1 * (h0 + h1));
+        }
+        if (k >= 2) {
+            double hm2 = z[k - 1] - z[k - 2], hm1 = z[k] - z[k - 1];
+            e[WF_Z_D1B + 0] = hm1 / (hm2 * (hm2 + hm1));
+            e[WF_Z_D1B + 1] = -(hm2 + hm1) / (hm2 * hm1);
+            e[WF_Z_D1B + 2] = (hm2 + 2 * hm1) / (hm1 * (hm2 + hm1));
+        }
+        if (k >= 1 && k <= Nz - 2) {
+            double hm1 = z[k] - z[k - 1], h0 = z[k + 1] - z[k];
+            e[WF_Z_D2M + 0] = 2 / (hm1 * (hm1 + h0));
+            e[WF_Z_D2M + 1] = -2 / (hm1 * h0);
+            e[WF_Z_D2M + 2] = 2 / (h0 * (hm1 + h0));
+            e[WF_Z_DZC] = 1 / (hm1 + h0);
+        }
+        if (k == 0) {
+            double h0 = z[1] - z[0], h1 = z[2] - z[1], h2 = z[3] - z[2];
+            e[WF_Z_D2F + 0] = (6 * h0 + 4 * h1 + 2 * h2) / (h0 * (h0 + h1) * (h0 + h1 + h2));
+            e[WF_Z_D2F + 1] = -(4 * h0 + 4 * h1 + 2 * h2) / (h0 * h1 * (h1 + h2));
+            e[WF_Z_D2F + 2] = (4 * h0 + 2 * h1 + 2 * h2) / (h1 * h2 * (h0 + h1));
+            e[WF_Z_D2F + 3] = -(4 * h0 + 2 * h1) / (h2 * (h1 + h2) * (h0 + h1 + h2));
+        }
+        if (k == Nz - 1) {
+            double hm1 = z[k] - z[k - 1], hm2 = z[k - 1] - z[k - 2], hm3 = z[k - 2] - z[k - 3];
+            e[WF_Z_D2B + 0] = -(2 * hm2 + 4 * hm1) / (hm3 * (hm3 + hm2) * (hm3 + hm2 + hm1));
+            e[WF_Z_D2B + 1] = (2 * hm3 + 2 * hm2 + 4 * hm1) / (hm3 * hm2 * (hm2 + hm1));
+            e[WF_Z_D2B + 2] = -(2 * hm3 + 4 * hm2 + 4 * hm1) / (hm2 * hm1 * (hm3 + hm2));
+            e[WF_Z_D2B + 3] = (2 * hm3 + 4 * hm2 + 6 * hm1) / (hm1 * (hm2 + hm1) * (hm3 + hm2 + hm1));
+        }
+    }
     if ((rc = wf_upload(&s->z, z, Nz))) return rc;
     if ((rc = wf_upload(&s->cd2, cd2.data(), Nz))) return rc;
+    if ((rc = wf_upload(&s->zc, zc.data(), zc.size()))) return rc;
     if ((rc = wf_upload(&s->topo, P->topography, s->plane))) return rc;
     if ((rc = wf_upload(&s->x, P->x, Nx))) return rc;
     if ((rc = wf_upload(&s->y, P->y, Ny))) return rc;
@@ -296,6 +336,15 @@ static int wf_create_impl(wf_solver *s, const Parameters *P) {
     c.exchange_on = (P->alpha_s * P->sigma_s != 0.0);
     c.z = s->z;
     c.cd2 = s->cd2;
+    c.zc = s->zc;
+    c.inv2dx = 1.0 / (2 * P->dx);
+    c.inv2dy = 1.0 / (2 * P->dy);
+    c.invdx2 = 1.0 / (P->dx * P->dx);
+    c.invdy2 = 1.0 / (P->dy * P->dy);
+    c.inv_cp = 1.0 / P->c_p;
+    c.inv_Trho = 1.0 / (P->T_inf * P->rho_inf);
+    c.inv_mu = 1.0 / P->mu;
+    c.inv_Pr = 1.0 / P->Pr;
     c.topo = s->topo;
     c.cut = s->cut;
     c.nzy = s->nzy;
@@ -407,7 +456,7 @@ extern "C" int wf_solver_destroy(wf_solver *s) {
     if (s == NULL) return WF_OK;
     if (s->stream) (void)hipStreamSynchronize(s->stream);
     for (int q = 0; q < 4; q++) (void)hipFree(s->state[q]);
-    void *bufs[] = {s->derived, s->p, s->A, s->L, s->RU, s->pmax, s->F, s->Ptop, s->z, s->cd2, s->topo, s->kx2, s->ky2, s->ta,
+    void *bufs[] = {s->zc, s->derived, s->p, s->A, s->L, s->RU, s->pmax, s->F, s->Ptop, s->z, s->cd2, s->topo, s->kx2, s->ky2, s->ta,
                     s->tc, s->beta, s->x, s->y, s->cut, s->nzy, s->radx, s->rady, s->twx, s->twy, s->scratch,
                     s->Tsrc, s->rho, s->ctl};
     for (size_t q = 0; q < sizeof(bufs) / sizeof(bufs[0]); q++)

@@ -6,86 +6,140 @@
  * reads at a neighbour: mu_sgs, the six distinct entries of the modified strain
  * tensor, div_U, and u_tau on the k = 0 plane (src/cu/pde.cu:632-772).
  * Pass 2 (k_stage) evaluates the full RHS of u, v, w, T, Y at a node
- * (src/cu/pde.cu:139-787 plus src/cu/turbulence.cu:255-454), applies the
- * column conditions of src/cu/pde.cu:14-99 by evaluating the raw RHS of the
- * two nodes above a cut node / below the top node in the same thread, and
- * writes the RK combination (src/cu/utils.cu:14-21, src/cu/solver.cu:14-39)
- * instead of storing the stage derivative.
+ * (src/cu/pde.cu:139-787 plus src/cu/turbulence.cu:255-454) and writes the RK
+ * combination (src/cu/utils.cu:14-21, src/cu/solver.cu:14-39) instead of
+ * storing the stage derivative; k_stage_dup does the same for the duplicate
+ * periodic planes and k_stage_columns completes the cut and top nodes of every
+ * column (src/cu/pde.cu:14-99).
+ *
+ * Arithmetic: the reference's difference formulas with their constant
+ * divisors (2 dx, dx^2, the non-uniform z weights, c_p, Pr, ...) taken as
+ * precomputed reciprocals and weights, and multiply-adds contracted to FMAs.
+ * Results agree with the reference's operation order to a few units in the
+ * last place; the data-dependent division by T is kept once per node.
  */
 #include "wf_common.h"
 #include "wf_kernels.h"
 
+/* 32-bit element offsets from a uniform base (global loads with a scalar base
+ * and a 32-bit lane offset; the solver limits Nx*Ny*Nz to 2^28). */
+__device__ __forceinline__ double wf_ld(const double *base, unsigned q) {
+    return *(const double *)((const char *)base + (q << 3));
+}
+
+/* Heat source q of the temperature RHS (src/cu/pde.cu:663-699): combustion
+ * H_C Y K(T)/c_p minus the gas-solid exchange, with the gas/solid temperatures
+ * of the fuel zone k <= Nz_Y (the exchange term is formed only when
+ * alpha_s sigma_s != 0, see wf_heat_source_y). */
+__device__ __forceinline__ double wf_heat_q(const WfConst &c, int k, int nzy, double T, double T_kp1, double Ynode,
+                                            double K_T, double dz_k, double *Yout) {
+#pragma clang fp contract(fast)
+    double T_gas = c.T_inf, T_solid = c.T_inf, Y = 0.0;
+    if (k <= nzy) {
+        if (k < nzy)
+            Y = Ynode;
+        else
+            T_gas = T_kp1;
+        if (Y > 0.0)
+            T_solid = T;
+        else
+            T_gas = T;
+    }
+    double q = c.H_C * Y * K_T * c.inv_cp;
+    if (c.exchange_on) {
+        double h_c = c.h_c * dz_k * pow(fabs(T_gas - T_solid) / dz_k, 1.0 / 4.0);
+        q = q - h_c * c.alpha_s * c.sigma_s * (T_gas - T_solid) / (c.c_p * c.T_inf * c.rho_inf / T_gas);
+    }
+    *Yout = Y;
+    return q;
+}
+
 /* ---------------------------------------------------------------- pass 1 ---- */
 
 __global__ __launch_bounds__(256) void k_derived(WfConst c, WfState X, WfDerived D) {
+#pragma clang fp contract(fast)
     int i, j, k;
     wf_tile_ijk(c, &i, &j, &k);
     if (i >= c.Nx || j >= c.Ny) return;
     const int Nx = c.Nx, Ny = c.Ny, Nz = c.Nz;
-    const double *z = c.z;
-    int im1 = wf_wrap_m(i, 1, Nx), ip1 = wf_wrap_p(i, 1, Nx);
-    int jm1 = wf_wrap_m(j, 1, Ny), jp1 = wf_wrap_p(j, 1, Ny);
-    long q = wf_q(c, i, j, k);
-    long qxm = wf_q(c, im1, j, k), qxp = wf_q(c, ip1, j, k);
-    long qym = wf_q(c, i, jm1, k), qyp = wf_q(c, i, jp1, k);
-    const long P = c.plane;
-    double u = X.u[q], v = X.v[q], w = X.w[q], T = X.T[q];
-    double ux = (X.u[qxp] - X.u[qxm]) / (2 * c.dx), uy = (X.u[qyp] - X.u[qym]) / (2 * c.dy);
-    double vx = (X.v[qxp] - X.v[qxm]) / (2 * c.dx), vy = (X.v[qyp] - X.v[qym]) / (2 * c.dy);
-    double wx = (X.w[qxp] - X.w[qxm]) / (2 * c.dx), wy = (X.w[qyp] - X.w[qym]) / (2 * c.dy);
-    double Txm = X.T[qxm], Txp = X.T[qxp], Tym = X.T[qym], Typ = X.T[qyp];
-    double Tx = (Txp - Txm) / (2.0 * c.dx), Ty = (Typ - Tym) / (2.0 * c.dy);
-    double Txx = (Txp - 2.0 * T + Txm) / (c.dx * c.dx);
-    double Tyy = (Typ - 2.0 * T + Tym) / (c.dy * c.dy);
+    const double *zk = c.zc + WF_ZC * k;
+    const int im1 = wf_wrap_m(i, 1, Nx), ip1 = wf_wrap_p(i, 1, Nx);
+    const int jm1 = wf_wrap_m(j, 1, Ny), jp1 = wf_wrap_p(j, 1, Ny);
+    const unsigned P = (unsigned)c.plane;
+    const unsigned q = (unsigned)wf_q(c, i, j, k);
+    const unsigned qxm = (unsigned)wf_q(c, im1, j, k), qxp = (unsigned)wf_q(c, ip1, j, k);
+    const unsigned qym = (unsigned)wf_q(c, i, jm1, k), qyp = (unsigned)wf_q(c, i, jp1, k);
+    /* z-neighbours k-3..k+3, clamped to the grid (unused ones are not read) */
+    const unsigned km1 = (k >= 1) ? q - P : q, km2 = (k >= 2) ? q - 2 * P : q, km3 = (k >= 3) ? q - 3 * P : q;
+    const unsigned kp1 = (k <= Nz - 2) ? q + P : q, kp2 = (k <= Nz - 3) ? q + 2 * P : q,
+                   kp3 = (k <= Nz - 4) ? q + 3 * P : q;
+    const double u = wf_ld(X.u, q), v = wf_ld(X.v, q), w = wf_ld(X.w, q), T = wf_ld(X.T, q);
+    const double ux = (wf_ld(X.u, qxp) - wf_ld(X.u, qxm)) * c.inv2dx, uy = (wf_ld(X.u, qyp) - wf_ld(X.u, qym)) * c.inv2dy;
+    const double vx = (wf_ld(X.v, qxp) - wf_ld(X.v, qxm)) * c.inv2dx, vy = (wf_ld(X.v, qyp) - wf_ld(X.v, qym)) * c.inv2dy;
+    const double wx = (wf_ld(X.w, qxp) - wf_ld(X.w, qxm)) * c.inv2dx, wy = (wf_ld(X.w, qyp) - wf_ld(X.w, qym)) * c.inv2dy;
+    const double Txm = wf_ld(X.T, qxm), Txp = wf_ld(X.T, qxp), Tym = wf_ld(X.T, qym), Typ = wf_ld(X.T, qyp);
+    const double Tx = (Txp - Txm) * c.inv2dx, Ty = (Typ - Tym) * c.inv2dy;
+    const double Txx = (Txp - 2.0 * T + Txm) * c.invdx2;
+    const double Tyy = (Typ - 2.0 * T + Tym) * c.invdy2;
+    const double u1 = wf_ld(X.u, kp1), u2 = wf_ld(X.u, kp2), um1 = wf_ld(X.u, km1), um2 = wf_ld(X.u, km2);
+    const double v1 = wf_ld(X.v, kp1), v2 = wf_ld(X.v, kp2), vm1 = wf_ld(X.v, km1), vm2 = wf_ld(X.v, km2);
+    const double w1 = wf_ld(X.w, kp1), w2 = wf_ld(X.w, kp2), wm1 = wf_ld(X.w, km1), wm2 = wf_ld(X.w, km2);
+    const double T1 = wf_ld(X.T, kp1), T2z = wf_ld(X.T, kp2), T3z = wf_ld(X.T, kp3);
+    const double Tm1 = wf_ld(X.T, km1), Tm2 = wf_ld(X.T, km2), Tm3 = wf_ld(X.T, km3);
     double uz, vz, wz, Tz, Tzz, T_kp1 = 0.0;
-    if (k == 0) {
-        double h0 = z[1] - z[0], h1 = z[2] - z[1], h2 = z[3] - z[2];
-        uz = wf_d1f(u, X.u[q + P], X.u[q + 2 * P], h0, h1);
-        vz = wf_d1f(v, X.v[q + P], X.v[q + 2 * P], h0, h1);
-        wz = wf_d1f(w, X.w[q + P], X.w[q + 2 * P], h0, h1);
-        T_kp1 = X.T[q + P];
-        double T2 = X.T[q + 2 * P], T3 = X.T[q + 3 * P];
-        Tz = wf_d1f(T, T_kp1, T2, h0, h1);
-        Tzz = wf_d2f(T, T_kp1, T2, T3, h0, h1, h2);
-    } else if (k == Nz - 1) {
-        double hm1 = z[k] - z[k - 1], hm2 = z[k - 1] - z[k - 2], hm3 = z[k - 2] - z[k - 3];
-        uz = wf_d1b(X.u[q - 2 * P], X.u[q - P], u, hm2, hm1);
-        vz = wf_d1b(X.v[q - 2 * P], X.v[q - P], v, hm2, hm1);
-        wz = wf_d1b(X.w[q - 2 * P], X.w[q - P], w, hm2, hm1);
-        double Tm1 = X.T[q - P], Tm2 = X.T[q - 2 * P], Tm3 = X.T[q - 3 * P];
-        Tz = wf_d1b(Tm2, Tm1, T, hm2, hm1);
-        Tzz = wf_d2b(Tm3, Tm2, Tm1, T, hm3, hm2, hm1);
+    if (k == 0) { /* one-sided (src/cu/pde.cu:363-365, 497-500) */
+        const double *f = zk + WF_Z_D1F, *g = zk + WF_Z_D2F;
+        uz = f[0] * u + f[1] * u1 + f[2] * u2;
+        vz = f[0] * v + f[1] * v1 + f[2] * v2;
+        wz = f[0] * w + f[1] * w1 + f[2] * w2;
+        T_kp1 = T1;
+        Tz = f[0] * T + f[1] * T1 + f[2] * T2z;
+        Tzz = g[0] * T + g[1] * T1 + g[2] * T2z + g[3] * T3z;
+    } else if (k == Nz - 1) { /* src/cu/pde.cu:383-385, 539-542 */
+        const double *b = zk + WF_Z_D1B, *g = zk + WF_Z_D2B;
+        uz = b[0] * um2 + b[1] * um1 + b[2] * u;
+        vz = b[0] * vm2 + b[1] * vm1 + b[2] * v;
+        wz = b[0] * wm2 + b[1] * wm1 + b[2] * w;
+        Tz = b[0] * Tm2 + b[1] * Tm1 + b[2] * T;
+        Tzz = g[0] * Tm3 + g[1] * Tm2 + g[2] * Tm1 + g[3] * T;
     } else {
-        double hm1 = z[k] - z[k - 1], h0 = z[k + 1] - z[k];
-        uz = (X.u[q + P] - X.u[q - P]) / (hm1 + h0);
-        vz = (X.v[q + P] - X.v[q - P]) / (hm1 + h0);
-        wz = (X.w[q + P] - X.w[q - P]) / (hm1 + h0);
-        T_kp1 = X.T[q + P];
-        double Tm1 = X.T[q - P];
-        Tz = (T_kp1 - Tm1) / (hm1 + h0);
-        Tzz = wf_d2m(Tm1, T, T_kp1, hm1, h0);
+        const double dzc = zk[WF_Z_DZC], *m = zk + WF_Z_D2M;
+        uz = (u1 - um1) * dzc;
+        vz = (v1 - vm1) * dzc;
+        wz = (w1 - wm1) * dzc;
+        T_kp1 = T1;
+        Tz = (T1 - Tm1) * dzc;
+        Tzz = m[0] * Tm1 + m[1] * T + m[2] * T1;
     }
-    double lap_T = Txx + Tyy + Tzz;
-    double rho = c.T_inf * c.rho_inf / T;
-    double s11 = 2 * ux / 3 - (vy + wz) / 3;
-    double s22 = 2 * vy / 3 - (ux + wz) / 3;
-    double s33 = 2 * wz / 3 - (ux + vy) / 3;
-    double s12 = 0.5 * (uy + vx);
-    double s13 = 0.5 * (uz + wx);
-    double s23 = 0.5 * (wy + vz);
-    double mod_S = sqrt(2.0 * (ux * ux + vy * vy + wz * wz) + (uz + wx) * (uz + wx) + (vx + uy) * (vx + uy) +
-                        (wy + vz) * (wy + vz));
-    double mu_sgs = rho * c.cd2[k] * mod_S;
+    const double lap_T = Txx + Tyy + Tzz;
+    const double invT = 1.0 / T;
+    const double rho = c.T_inf * c.rho_inf * invT;
+    const double inv_rho = T * c.inv_Trho;
+    const double third = 1.0 / 3.0;
+    const double s11 = (2 * ux) * third - (vy + wz) * third;
+    const double s22 = (2 * vy) * third - (ux + wz) * third;
+    const double s33 = (2 * wz) * third - (ux + vy) * third;
+    const double s12 = 0.5 * (uy + vx);
+    const double s13 = 0.5 * (uz + wx);
+    const double s23 = 0.5 * (wy + vz);
+    const double mod_S = sqrt(2.0 * (ux * ux + vy * vy + wz * wz) + (uz + wx) * (uz + wx) + (vx + uy) * (vx + uy) +
+                              (wy + vz) * (wy + vz));
+    const double mu_sgs = rho * c.cd2[k] * mod_S;
     /* T_RHS_tmp of src/cu/pde.cu:706 (needed for div_U) */
-    int col = i + Nx * j;
+    const int col = i + Nx * j;
+    const int nzy = c.nzy[col];
     double mu = c.mu, kT = c.kT, dk = 0.0;
-    double q_src = wf_heat_source(c, col, i, j, k, T, T_kp1, X.Y);
+    const double K_T = (T > c.T_pc) ? c.A * exp(-c.T_act * invT) : 0.0;
+    const double Ynode = (k < nzy) ? wf_ld(X.Y, q) : 0.0;
+    const double dz_k = (k < Nz - 1) ? c.z[k + 1] - c.z[k] : c.z[k] - c.z[k - 1];
+    double Y;
+    const double q_src = wf_heat_q(c, k, nzy, T, T_kp1, Ynode, K_T, dz_k, &Y);
     if (c.suth_mu == 1) mu = 1.716e-5 * pow(T / 273.15, 1.5) * (273.15 + 110.4) / (T + 110.4);
     if (c.suth_k == 1) wf_sutherland_k(T, &kT, &dk);
-    double nu = mu / rho;
-    double T2 = T * T, T3 = T * T * T;
-    double T_RHS_tmp = (dk + 12 * WF_SIGMA_D * c.delta * T2 * (Tx * Tx + Ty * Ty + Tz * Tz) +
-                        (kT + 4 * WF_SIGMA_D * c.delta * T3) * lap_T) / (rho * c.c_p) + q_src;
+    const double nu = mu * inv_rho;
+    const double T2 = T * T, T3 = T * T * T;
+    const double T_RHS_tmp = (dk + 12 * WF_SIGMA_D * c.delta * T2 * (Tx * Tx + Ty * Ty + Tz * Tz) +
+                              (kT + 4 * WF_SIGMA_D * c.delta * T3) * lap_T) * inv_rho * c.inv_cp + q_src;
     D.mu[q] = mu_sgs;
     D.s11[q] = s11;
     D.s12[q] = s12;
@@ -93,21 +147,16 @@ __global__ __launch_bounds__(256) void k_derived(WfConst c, WfState X, WfDerived
     D.s22[q] = s22;
     D.s23[q] = s23;
     D.s33[q] = s33;
-    D.divU[q] = nu * T_RHS_tmp / T;
+    D.divU[q] = nu * T_RHS_tmp * invT;
     if (k == 0) {
-        double a1 = mu * (uz + wx * 0.0), a2 = mu * (vz + wy * 0.0);
-        double tau_w = sqrt(a1 * a1 + a2 * a2);
-        D.utau[col] = sqrt(tau_w / rho);
+        const double a1 = mu * (uz + wx * 0.0), a2 = mu * (vz + wy * 0.0);
+        const double tau_w = sqrt(a1 * a1 + a2 * a2);
+        D.utau[col] = sqrt(tau_w * inv_rho);
     }
 }
 
 /* ---------------------------------------------------------------- pass 2 ---- */
 
-/* 32-bit element offsets from a uniform base (global loads with a scalar base
- * and a 32-bit lane offset; the solver limits Nx*Ny*Nz to 2^28). */
-__device__ __forceinline__ double wf_ld(const double *base, unsigned q) {
-    return *(const double *)((const char *)base + (q << 3));
-}
 
 /* LDS halo tiles of one plane of a WF_TILE_X x WF_TILE_Y node tile. */
 #define WF_SH 2 /* state halo (upwind stencils reach i +- 2) */
@@ -165,11 +214,8 @@ __device__ __forceinline__ void wf_tile_store(double *dst, const double *v, int 
     }
 }
 
-/* Raw RHS of u, v, w, T, Y (RHS + SGS, before any boundary condition) at an
- * interior node 1 <= k <= Nz-2 of the tile: neighbours in x,y from the LDS
- * tiles, the node itself and its z-neighbours from registers.  Every
- * expression is the reference's (src/cu/pde.cu:139-787,
- * src/cu/turbulence.cu:255-454), evaluated in its order. */
+/* The node's own column: u, v, w, T at k-2..k+2 (index 2 = k) and the
+ * derived fields the SGS divergence reads above and below. */
 struct WfColumnVals {
     double s[4][5];   /* u, v, w, T at k-2, k-1, k, k+1, k+2 (index 2 = k) */
     double mu[3], s13[3], s23[3], s33[3], dU[3]; /* k-1, k, k+1 */
@@ -177,290 +223,208 @@ struct WfColumnVals {
     double Y, utau, topo;
 };
 
-__device__ __forceinline__ void wf_rhs_tile(const WfConst &c, const WfStageTiles &L, const WfColumnVals &V, int tx,
-                                            int ty, int col, int k, int nzy, double r[5]) {
+/* Derived-field indices of the accessors (the first seven are the LDS tiles). */
+enum { WF_D_MU = 0, WF_D_S11, WF_D_S12, WF_D_S13, WF_D_S22, WF_D_S23, WF_D_DIVU, WF_D_S33 };
+
+/* Stencil access of the RHS at node (i, j, k), 1 <= k <= Nz-2:
+ *   st(f, di, dj)  state f (u, v, w, T) at (i+di, j+dj, k), periodic neighbours
+ *   sz(f, dk)      state f at (i, j, k+dk), |dk| <= 2 (reads clamped to the grid)
+ *   dv(f, di, dj)  derived f (WF_D_*) at (i+di, j+dj, k), |di| + |dj| = 1
+ *   dz(f, dk)      derived f at (i, j, k+dk), |dk| <= 1
+ *   ut(di, dj), tp(di, dj)  u_tau and topography of column (i+di, j+dj)
+ *   Y()            fuel fraction of the node (valid where k < Nz_Y) */
+struct WfTileAcc {
+    const WfStageTiles &L;
+    const WfColumnVals &V;
+    int sc, dc;
+    __device__ __forceinline__ double st(int f, int di, int dj) const { return L.st[f][sc + di + dj * WF_SW]; }
+    __device__ __forceinline__ double sz(int f, int dk) const { return V.s[f][2 + dk]; }
+    __device__ __forceinline__ double dv(int f, int di, int dj) const { return L.dv[f][dc + di + dj * WF_DW]; }
+    __device__ __forceinline__ double dz(int f, int dk) const {
+        switch (f) {
+            case WF_D_MU: return V.mu[1 + dk];
+            case WF_D_S13: return V.s13[1 + dk];
+            case WF_D_S23: return V.s23[1 + dk];
+            case WF_D_S33: return V.s33[1 + dk];
+            case WF_D_DIVU: return V.dU[1 + dk];
+            case WF_D_S11: return V.s11;
+            case WF_D_S12: return V.s12;
+            default: return V.s22;
+        }
+    }
+    __device__ __forceinline__ double ut(int di, int dj) const {
+        return (di == 0 && dj == 0) ? V.utau : L.ut[dc + di + dj * WF_DW];
+    }
+    __device__ __forceinline__ double tp(int di, int dj) const {
+        return (di == 0 && dj == 0) ? V.topo : L.tp[dc + di + dj * WF_DW];
+    }
+    __device__ __forceinline__ double Y() const { return V.Y; }
+};
+
+struct WfGlobalAcc {
+    const WfConst &c;
+    const WfState &X;
+    const WfDerived &D;
+    int i, j, k;
+    __device__ __forceinline__ int ni(int di) const {
+        return (di > 0) ? wf_wrap_p(i, di, c.Nx) : (di < 0) ? wf_wrap_m(i, -di, c.Nx) : i;
+    }
+    __device__ __forceinline__ int nj(int dj) const {
+        return (dj > 0) ? wf_wrap_p(j, dj, c.Ny) : (dj < 0) ? wf_wrap_m(j, -dj, c.Ny) : j;
+    }
+    __device__ __forceinline__ const double *sfield(int f) const {
+        return (f == 0) ? X.u : (f == 1) ? X.v : (f == 2) ? X.w : X.T;
+    }
+    __device__ __forceinline__ const double *dfield(int f) const {
+        switch (f) {
+            case WF_D_MU: return D.mu;
+            case WF_D_S11: return D.s11;
+            case WF_D_S12: return D.s12;
+            case WF_D_S13: return D.s13;
+            case WF_D_S22: return D.s22;
+            case WF_D_S23: return D.s23;
+            case WF_D_DIVU: return D.divU;
+            default: return D.s33;
+        }
+    }
+    __device__ __forceinline__ double st(int f, int di, int dj) const {
+        return wf_ld(sfield(f), (unsigned)wf_q(c, ni(di), nj(dj), k));
+    }
+    __device__ __forceinline__ double sz(int f, int dk) const {
+        int kk = k + dk;
+        kk = (kk < 0) ? 0 : (kk > c.Nz - 1) ? c.Nz - 1 : kk;
+        return wf_ld(sfield(f), (unsigned)wf_q(c, i, j, kk));
+    }
+    __device__ __forceinline__ double dv(int f, int di, int dj) const {
+        return wf_ld(dfield(f), (unsigned)wf_q(c, ni(di), nj(dj), k));
+    }
+    __device__ __forceinline__ double dz(int f, int dk) const { return wf_ld(dfield(f), (unsigned)wf_q(c, i, j, k + dk)); }
+    __device__ __forceinline__ double ut(int di, int dj) const { return D.utau[ni(di) + c.Nx * nj(dj)]; }
+    __device__ __forceinline__ double tp(int di, int dj) const { return c.topo[ni(di) + c.Nx * nj(dj)]; }
+    __device__ __forceinline__ double Y() const {
+        return (k < c.Nzm) ? wf_ld(X.Y, (unsigned)wf_q(c, i, j, k)) : 0.0;
+    }
+};
+
+/* Raw RHS of u, v, w, T, Y (RHS + SGS, before any boundary condition) at an
+ * interior node 1 <= k <= Nz-2 (src/cu/pde.cu:139-787,
+ * src/cu/turbulence.cu:255-454). */
+template <class Acc>
+__device__ __forceinline__ void wf_rhs_node(const WfConst &c, const Acc &a, int k, int nzy, double r[5]) {
+#pragma clang fp contract(fast)
     const int Nz = c.Nz;
-    const double *z = c.z;
-    const double hm1 = z[k] - z[k - 1], h0 = z[k + 1] - z[k];
-    const double g1 = (k <= Nz - 3) ? z[k + 2] - z[k + 1] : 0.0;
-    const double g2 = (k >= 2) ? z[k - 1] - z[k - 2] : 0.0;
-    const double u = V.s[0][2], v = V.s[1][2], w = V.s[2][2], T = V.s[3][2];
+    const double *zk = c.zc + WF_ZC * k;
+    const double *d1f = zk + WF_Z_D1F, *d1b = zk + WF_Z_D1B, *d2m = zk + WF_Z_D2M;
+    const double dzc = zk[WF_Z_DZC];
+    const double u = a.sz(0, 0), v = a.sz(1, 0), w = a.sz(2, 0), T = a.sz(3, 0);
     const double u_plu = (u > 0.0) ? u : 0.0, u_min = (u < 0.0) ? u : 0.0;
     const double v_plu = (v > 0.0) ? v : 0.0, v_min = (v < 0.0) ? v : 0.0;
     const double w_plu = (w > 0.0) ? w : 0.0, w_min = (w < 0.0) ? w : 0.0;
-    const int sc = (ty + WF_SH) * WF_SW + (tx + WF_SH);
     double adv[4], lap[4];
     double Tx = 0.0, Ty = 0.0, Tz = 0.0, Txx = 0.0, Tyy = 0.0, Tzz = 0.0, T_kp1 = 0.0;
 #pragma unroll
     for (int f = 0; f < 4; f++) {
-        const double *A = L.st[f];
-        const double a0 = V.s[f][2];
-        const double am1 = A[sc - 1], am2 = A[sc - 2], ap1 = A[sc + 1], ap2 = A[sc + 2];
-        const double bm1 = A[sc - WF_SW], bm2 = A[sc - 2 * WF_SW], bp1 = A[sc + WF_SW], bp2 = A[sc + 2 * WF_SW];
-        const double cm2 = V.s[f][0], cm1 = V.s[f][1], cp1 = V.s[f][3], cp2 = V.s[f][4];
-        const double d_im = (3 * a0 - 4 * am1 + am2) / (2 * c.dx);
-        const double d_ip = (-3 * a0 + 4 * ap1 - ap2) / (2 * c.dx);
-        const double d_jm = (3 * a0 - 4 * bm1 + bm2) / (2 * c.dy);
-        const double d_jp = (-3 * a0 + 4 * bp1 - bp2) / (2 * c.dy);
-        const double d_km = (k <= 1) ? wf_d1f(a0, cp1, cp2, h0, g1) : wf_d1b(cm2, cm1, a0, g2, hm1);
-        const double d_kp = (k >= Nz - 2) ? wf_d1b(cm2, cm1, a0, g2, hm1) : wf_d1f(a0, cp1, cp2, h0, g1);
-        const double dxx = (ap1 - 2.0 * a0 + am1) / (c.dx * c.dx);
-        const double dyy = (bp1 - 2.0 * a0 + bm1) / (c.dy * c.dy);
-        const double dzz = wf_d2m(cm1, a0, cp1, hm1, h0);
+        const double a0 = a.sz(f, 0);
+        const double am1 = a.st(f, -1, 0), am2 = a.st(f, -2, 0), ap1 = a.st(f, 1, 0), ap2 = a.st(f, 2, 0);
+        const double bm1 = a.st(f, 0, -1), bm2 = a.st(f, 0, -2), bp1 = a.st(f, 0, 1), bp2 = a.st(f, 0, 2);
+        const double cm2 = a.sz(f, -2), cm1 = a.sz(f, -1), cp1 = a.sz(f, 1), cp2 = a.sz(f, 2);
+        /* second-order upwind differences (src/cu/pde.cu:355-451) */
+        const double d_im = (3 * a0 - 4 * am1 + am2) * c.inv2dx;
+        const double d_ip = (-3 * a0 + 4 * ap1 - ap2) * c.inv2dx;
+        const double d_jm = (3 * a0 - 4 * bm1 + bm2) * c.inv2dy;
+        const double d_jp = (-3 * a0 + 4 * bp1 - bp2) * c.inv2dy;
+        const double fwd = d1f[0] * a0 + d1f[1] * cp1 + d1f[2] * cp2;
+        const double bwd = d1b[0] * cm2 + d1b[1] * cm1 + d1b[2] * a0;
+        const double d_km = (k <= 1) ? fwd : bwd;
+        const double d_kp = (k >= Nz - 2) ? bwd : fwd;
+        const double dxx = (ap1 - 2.0 * a0 + am1) * c.invdx2;
+        const double dyy = (bp1 - 2.0 * a0 + bm1) * c.invdy2;
+        const double dzz = d2m[0] * cm1 + d2m[1] * a0 + d2m[2] * cp1;
         const double ax = u_plu * d_im + u_min * d_ip;
         const double ay = v_plu * d_jm + v_min * d_jp;
         const double az = w_plu * d_km + w_min * d_kp;
         adv[f] = ax + ay + az;
         lap[f] = dxx + dyy + dzz;
         if (f == 3) {
-            Tx = (ap1 - am1) / (2.0 * c.dx);
-            Ty = (bp1 - bm1) / (2.0 * c.dy);
-            Tz = (cp1 - cm1) / (hm1 + h0);
+            Tx = (ap1 - am1) * c.inv2dx;
+            Ty = (bp1 - bm1) * c.inv2dy;
+            Tz = (cp1 - cm1) * dzc;
             Txx = dxx;
             Tyy = dyy;
             Tzz = dzz;
             T_kp1 = cp1;
         }
     }
-    const double rho = c.T_inf * c.rho_inf / T;
-    /* heat source and fuel zone (src/cu/pde.cu:663-699) */
-    double K_T = wf_rate(c, T);
-    double T_gas = c.T_inf, T_solid = c.T_inf, Y = 0.0;
-    if (k <= nzy) {
-        if (k < nzy)
-            Y = V.Y;
-        else
-            T_gas = T_kp1;
-        if (Y > 0.0)
-            T_solid = T;
-        else
-            T_gas = T;
-    }
-    double q_src = c.H_C * Y * K_T / c.c_p;
-    if (c.exchange_on) {
-        double dz_k = h0;
-        double h_c = c.h_c * dz_k * pow(fabs(T_gas - T_solid) / dz_k, 1.0 / 4.0);
-        q_src = q_src - h_c * c.alpha_s * c.sigma_s * (T_gas - T_solid) / (c.c_p * c.T_inf * c.rho_inf / T_gas);
-    }
+    const double invT = 1.0 / T;
+    const double rho = c.T_inf * c.rho_inf * invT;
+    const double inv_rho = T * c.inv_Trho;
+    const double K_T = (T > c.T_pc) ? c.A * exp(-c.T_act * invT) : 0.0;
+    double Y;
+    const double q_src = wf_heat_q(c, k, nzy, T, T_kp1, a.Y(), K_T, c.z[k + 1] - c.z[k], &Y);
     double mu = c.mu, kT = c.kT, dk = 0.0;
     if (c.suth_mu == 1) mu = 1.716e-5 * pow(T / 273.15, 1.5) * (273.15 + 110.4) / (T + 110.4);
     if (c.suth_k == 1) wf_sutherland_k(T, &kT, &dk);
-    double nu = mu / rho;
-    double T2 = T * T, T3 = T * T * T;
-    double T_RHS_tmp = (dk + 12 * WF_SIGMA_D * c.delta * T2 * (Tx * Tx + Ty * Ty + Tz * Tz) +
-                        (kT + 4 * WF_SIGMA_D * c.delta * T3) * lap[3]) / (rho * c.c_p) + q_src;
-    double T_RHS;
-    if (c.temperature_convection == 0)
-        T_RHS = T_RHS_tmp - (u * Tx + v * Ty + w * Tz);
-    else
-        T_RHS = T_RHS_tmp - adv[3];
-    double mod_U = sqrt(u * u + v * v + w * w);
-    double F_x = -0.5 * c.C_d * c.alpha_s * c.sigma_s * Y * mod_U * u;
-    double F_y = -0.5 * c.C_d * c.alpha_s * c.sigma_s * Y * mod_U * v;
-    double F_z = c.g * (rho - c.rho_inf) / rho - 0.5 * c.C_d * c.alpha_s * c.sigma_s * Y * mod_U * w;
-    double u_RHS = nu * lap[0] - adv[0] + F_x;
-    double v_RHS = nu * lap[1] - adv[1] + F_y;
-    double w_RHS = nu * lap[2] - adv[2] + F_z;
+    const double nu = mu * inv_rho;
+    const double T2 = T * T, T3 = T * T * T;
+    const double T_RHS_tmp = (dk + 12 * WF_SIGMA_D * c.delta * T2 * (Tx * Tx + Ty * Ty + Tz * Tz) +
+                              (kT + 4 * WF_SIGMA_D * c.delta * T3) * lap[3]) * inv_rho * c.inv_cp + q_src;
+    const double T_RHS = (c.temperature_convection == 0) ? T_RHS_tmp - (u * Tx + v * Ty + w * Tz) : T_RHS_tmp - adv[3];
+    const double mod_U = sqrt(u * u + v * v + w * w);
+    const double drag = -0.5 * c.C_d * c.alpha_s * c.sigma_s * Y * mod_U;
+    const double F_x = drag * u;
+    const double F_y = drag * v;
+    const double F_z = c.g * (rho - c.rho_inf) * inv_rho + drag * w;
+    const double u_RHS = nu * lap[0] - adv[0] + F_x;
+    const double v_RHS = nu * lap[1] - adv[1] + F_y;
+    const double w_RHS = nu * lap[2] - adv[2] + F_z;
 
-    /* SGS divergence (src/cu/turbulence.cu:291-452) */
-    const int dc = (ty + WF_DH) * WF_DW + (tx + WF_DH);
-    double nu_t = c.mu / rho;
-    double ut = V.utau;
-    double fw = wf_fw(wf_zibm_t(c, V.topo, k), ut, nu_t);
-    double fw_ip = wf_fw(wf_zibm_t(c, L.tp[dc + 1], k), L.ut[dc + 1], nu_t);
-    double fw_im = wf_fw(wf_zibm_t(c, L.tp[dc - 1], k), L.ut[dc - 1], nu_t);
-    double fw_jp = wf_fw(wf_zibm_t(c, L.tp[dc + WF_DW], k), L.ut[dc + WF_DW], nu_t);
-    double fw_jm = wf_fw(wf_zibm_t(c, L.tp[dc - WF_DW], k), L.ut[dc - WF_DW], nu_t);
-    double fw_km = wf_fw(wf_zibm_t(c, V.topo, k - 1), ut, nu_t);
-    double fw_kp = wf_fw(wf_zibm_t(c, V.topo, k + 1), ut, nu_t);
-    const double *Mu = L.dv[0];
-    double mu_c = V.mu[1];
-    double mu_ip = Mu[dc + 1], mu_im = Mu[dc - 1], mu_jp = Mu[dc + WF_DW], mu_jm = Mu[dc - WF_DW];
-    double mu_km = V.mu[0], mu_kp = V.mu[2];
-    mu_c *= fw * fw;
-    mu_im *= fw_im * fw_im;
-    mu_ip *= fw_ip * fw_ip;
-    mu_jm *= fw_jm * fw_jm;
-    mu_jp *= fw_jp * fw_jp;
-    mu_km *= fw_km * fw_km;
-    mu_kp *= fw_kp * fw_kp;
-    double S11 = V.s11, S12 = V.s12, S13 = V.s13[1], S22 = V.s22, S23 = V.s23[1], S33 = V.s33[1];
-    const double *s11 = L.dv[1], *s12 = L.dv[2], *s13 = L.dv[3], *s22 = L.dv[4], *s23 = L.dv[5], *dU = L.dv[6];
-    double mu_x = (mu_ip - mu_im) / (2 * c.dx);
-    double mu_y = (mu_jp - mu_jm) / (2 * c.dy);
-    double S11_x = (s11[dc + 1] - s11[dc - 1]) / (2.0 * c.dx);
-    double S12_x = (s12[dc + 1] - s12[dc - 1]) / (2.0 * c.dx);
-    double S13_x = (s13[dc + 1] - s13[dc - 1]) / (2.0 * c.dx);
-    double S21_y = (s12[dc + WF_DW] - s12[dc - WF_DW]) / (2.0 * c.dy);
-    double S22_y = (s22[dc + WF_DW] - s22[dc - WF_DW]) / (2.0 * c.dy);
-    double S23_y = (s23[dc + WF_DW] - s23[dc - WF_DW]) / (2.0 * c.dy);
-    double dU_x = (dU[dc + 1] - dU[dc - 1]) / (2.0 * c.dx);
-    double dU_y = (dU[dc + WF_DW] - dU[dc - WF_DW]) / (2.0 * c.dy);
-    double mu_z = (mu_kp - mu_km) / (h0 + hm1);
-    double S31_z = (V.s13[2] - V.s13[0]) / (h0 + hm1);
-    double S32_z = (V.s23[2] - V.s23[0]) / (h0 + hm1);
-    double S33_z = (V.s33[2] - V.s33[0]) / (h0 + hm1);
-    double dU_z = (V.dU[2] - V.dU[0]) / (h0 + hm1);
-    double div_S_x = S11_x + S21_y + S31_z;
-    double div_S_y = S12_x + S22_y + S32_z;
-    double div_S_z = S13_x + S23_y + S33_z;
-    double sgs_x = -2 * (mu_x * S11 + mu_y * S12 + mu_z * S13 + mu_c * div_S_x) / rho;
-    double sgs_y = -2 * (mu_x * S12 + mu_y * S22 + mu_z * S23 + mu_c * div_S_y) / rho;
-    double sgs_z = -2 * (mu_x * S13 + mu_y * S23 + mu_z * S33 + mu_c * div_S_z) / rho;
-    double sgs_q = -(mu_x * Tx + mu_y * Ty + mu_z * Tz + mu_c * (Txx + Tyy + Tzz)) / (rho * c.Pr);
-    r[0] = u_RHS + (-sgs_x + dU_x / 3.0);
-    r[1] = v_RHS + (-sgs_y + dU_y / 3.0);
-    r[2] = w_RHS + (-sgs_z + dU_z / 3.0);
+    /* SGS divergence (src/cu/turbulence.cu:291-452): mu_sgs damped by
+     * fw^2 = (1 - exp(-z_ibm u_tau / (A_VD nu)))^2 at the node and its six
+     * neighbours, nu the node's own c.mu / rho */
+    const double inv_nu_t = rho * c.inv_mu;
+    const double ut0 = a.ut(0, 0), tp0 = a.tp(0, 0);
+    const double fw = wf_fw_inv(wf_zibm_t(c, tp0, k), ut0, inv_nu_t);
+    const double fw_ip = wf_fw_inv(wf_zibm_t(c, a.tp(1, 0), k), a.ut(1, 0), inv_nu_t);
+    const double fw_im = wf_fw_inv(wf_zibm_t(c, a.tp(-1, 0), k), a.ut(-1, 0), inv_nu_t);
+    const double fw_jp = wf_fw_inv(wf_zibm_t(c, a.tp(0, 1), k), a.ut(0, 1), inv_nu_t);
+    const double fw_jm = wf_fw_inv(wf_zibm_t(c, a.tp(0, -1), k), a.ut(0, -1), inv_nu_t);
+    const double fw_km = wf_fw_inv(wf_zibm_t(c, tp0, k - 1), ut0, inv_nu_t);
+    const double fw_kp = wf_fw_inv(wf_zibm_t(c, tp0, k + 1), ut0, inv_nu_t);
+    const double mu_c = a.dz(WF_D_MU, 0) * (fw * fw);
+    const double mu_ip = a.dv(WF_D_MU, 1, 0) * (fw_ip * fw_ip), mu_im = a.dv(WF_D_MU, -1, 0) * (fw_im * fw_im);
+    const double mu_jp = a.dv(WF_D_MU, 0, 1) * (fw_jp * fw_jp), mu_jm = a.dv(WF_D_MU, 0, -1) * (fw_jm * fw_jm);
+    const double mu_kp = a.dz(WF_D_MU, 1) * (fw_kp * fw_kp), mu_km = a.dz(WF_D_MU, -1) * (fw_km * fw_km);
+    const double S11 = a.dz(WF_D_S11, 0), S12 = a.dz(WF_D_S12, 0), S13 = a.dz(WF_D_S13, 0);
+    const double S22 = a.dz(WF_D_S22, 0), S23 = a.dz(WF_D_S23, 0), S33 = a.dz(WF_D_S33, 0);
+    const double mu_x = (mu_ip - mu_im) * c.inv2dx;
+    const double mu_y = (mu_jp - mu_jm) * c.inv2dy;
+    const double mu_z = (mu_kp - mu_km) * dzc;
+    const double S11_x = (a.dv(WF_D_S11, 1, 0) - a.dv(WF_D_S11, -1, 0)) * c.inv2dx;
+    const double S12_x = (a.dv(WF_D_S12, 1, 0) - a.dv(WF_D_S12, -1, 0)) * c.inv2dx;
+    const double S13_x = (a.dv(WF_D_S13, 1, 0) - a.dv(WF_D_S13, -1, 0)) * c.inv2dx;
+    const double S21_y = (a.dv(WF_D_S12, 0, 1) - a.dv(WF_D_S12, 0, -1)) * c.inv2dy;
+    const double S22_y = (a.dv(WF_D_S22, 0, 1) - a.dv(WF_D_S22, 0, -1)) * c.inv2dy;
+    const double S23_y = (a.dv(WF_D_S23, 0, 1) - a.dv(WF_D_S23, 0, -1)) * c.inv2dy;
+    const double dU_x = (a.dv(WF_D_DIVU, 1, 0) - a.dv(WF_D_DIVU, -1, 0)) * c.inv2dx;
+    const double dU_y = (a.dv(WF_D_DIVU, 0, 1) - a.dv(WF_D_DIVU, 0, -1)) * c.inv2dy;
+    const double S31_z = (a.dz(WF_D_S13, 1) - a.dz(WF_D_S13, -1)) * dzc;
+    const double S32_z = (a.dz(WF_D_S23, 1) - a.dz(WF_D_S23, -1)) * dzc;
+    const double S33_z = (a.dz(WF_D_S33, 1) - a.dz(WF_D_S33, -1)) * dzc;
+    const double dU_z = (a.dz(WF_D_DIVU, 1) - a.dz(WF_D_DIVU, -1)) * dzc;
+    const double div_S_x = S11_x + S21_y + S31_z;
+    const double div_S_y = S12_x + S22_y + S32_z;
+    const double div_S_z = S13_x + S23_y + S33_z;
+    const double sgs_x = -2 * (mu_x * S11 + mu_y * S12 + mu_z * S13 + mu_c * div_S_x) * inv_rho;
+    const double sgs_y = -2 * (mu_x * S12 + mu_y * S22 + mu_z * S23 + mu_c * div_S_y) * inv_rho;
+    const double sgs_z = -2 * (mu_x * S13 + mu_y * S23 + mu_z * S33 + mu_c * div_S_z) * inv_rho;
+    const double sgs_q = -(mu_x * Tx + mu_y * Ty + mu_z * Tz + mu_c * (Txx + Tyy + Tzz)) * inv_rho * c.inv_Pr;
+    const double third = 1.0 / 3.0;
+    r[0] = u_RHS + (-sgs_x + dU_x * third);
+    r[1] = v_RHS + (-sgs_y + dU_y * third);
+    r[2] = w_RHS + (-sgs_z + dU_z * third);
     r[3] = T_RHS - sgs_q;
     /* fuel (src/cu/pde.cu:672-675): -Y_f K(T) Y where the column stores fuel */
-    r[4] = (k < nzy) ? -c.Y_f * K_T * V.Y : 0.0;
-}
-
-/* Raw RHS of u, v, w, T (RHS + SGS, before any boundary condition) at an
- * interior plane 1 <= k <= Nz-2, every neighbour read from global memory
- * (used on the duplicate planes i = Nx-1, j = Ny-1).  The stencil terms of each field are folded
- * into its advection sum and Laplacian as soon as they are formed, so few
- * intermediate values are live at once; every expression is the reference's. */
-__device__ void wf_rhs_interior(const WfConst &c, const WfState &X, const WfDerived &D, int i, int j, int k,
-                                double out[4]) {
-    const int Nx = c.Nx, Ny = c.Ny, Nz = c.Nz;
-    const double *z = c.z;
-    const long P = c.plane;
-    int im1 = wf_wrap_m(i, 1, Nx), im2 = wf_wrap_m(i, 2, Nx), ip1 = wf_wrap_p(i, 1, Nx), ip2 = wf_wrap_p(i, 2, Nx);
-    int jm1 = wf_wrap_m(j, 1, Ny), jm2 = wf_wrap_m(j, 2, Ny), jp1 = wf_wrap_p(j, 1, Ny), jp2 = wf_wrap_p(j, 2, Ny);
-    long q = wf_q(c, i, j, k);
-    long qxm1 = wf_q(c, im1, j, k), qxm2 = wf_q(c, im2, j, k), qxp1 = wf_q(c, ip1, j, k), qxp2 = wf_q(c, ip2, j, k);
-    long qym1 = wf_q(c, i, jm1, k), qym2 = wf_q(c, i, jm2, k), qyp1 = wf_q(c, i, jp1, k), qyp2 = wf_q(c, i, jp2, k);
-    const double hm1 = z[k] - z[k - 1], h0 = z[k + 1] - z[k];
-    const double g1 = (k <= Nz - 3) ? z[k + 2] - z[k + 1] : 0.0;
-    const double g2 = (k >= 2) ? z[k - 1] - z[k - 2] : 0.0;
-    const double u = X.u[q], v = X.v[q], w = X.w[q], T = X.T[q];
-    const double u_plu = (u > 0.0) ? u : 0.0, u_min = (u < 0.0) ? u : 0.0;
-    const double v_plu = (v > 0.0) ? v : 0.0, v_min = (v < 0.0) ? v : 0.0;
-    const double w_plu = (w > 0.0) ? w : 0.0, w_min = (w < 0.0) ? w : 0.0;
-    double adv[4], lap[4];
-    double Tx = 0.0, Ty = 0.0, Tz = 0.0, Txx = 0.0, Tyy = 0.0, Tzz = 0.0, T_kp1 = 0.0;
-#pragma unroll
-    for (int f = 0; f < 4; f++) {
-        const double *A = (f == 0) ? X.u : (f == 1) ? X.v : (f == 2) ? X.w : X.T;
-        const double a0 = (f == 0) ? u : (f == 1) ? v : (f == 2) ? w : T;
-        const double am1 = A[qxm1], am2 = A[qxm2], ap1 = A[qxp1], ap2 = A[qxp2];
-        const double bm1 = A[qym1], bm2 = A[qym2], bp1 = A[qyp1], bp2 = A[qyp2];
-        const double cm1 = A[q - P], cp1 = A[q + P];
-        const double d_im = (3 * a0 - 4 * am1 + am2) / (2 * c.dx);
-        const double d_ip = (-3 * a0 + 4 * ap1 - ap2) / (2 * c.dx);
-        const double d_jm = (3 * a0 - 4 * bm1 + bm2) / (2 * c.dy);
-        const double d_jp = (-3 * a0 + 4 * bp1 - bp2) / (2 * c.dy);
-        double d_km, d_kp;
-        if (k <= 1)
-            d_km = wf_d1f(a0, cp1, A[q + 2 * P], h0, g1);
-        else
-            d_km = wf_d1b(A[q - 2 * P], cm1, a0, g2, hm1);
-        if (k >= Nz - 2)
-            d_kp = wf_d1b(A[q - 2 * P], cm1, a0, g2, hm1);
-        else
-            d_kp = wf_d1f(a0, cp1, A[q + 2 * P], h0, g1);
-        const double dxx = (ap1 - 2.0 * a0 + am1) / (c.dx * c.dx);
-        const double dyy = (bp1 - 2.0 * a0 + bm1) / (c.dy * c.dy);
-        const double dzz = wf_d2m(cm1, a0, cp1, hm1, h0);
-        const double ax = u_plu * d_im + u_min * d_ip;
-        const double ay = v_plu * d_jm + v_min * d_jp;
-        const double az = w_plu * d_km + w_min * d_kp;
-        adv[f] = ax + ay + az;
-        lap[f] = dxx + dyy + dzz;
-        if (f == 3) {
-            Tx = (ap1 - am1) / (2.0 * c.dx);
-            Ty = (bp1 - bm1) / (2.0 * c.dy);
-            Tz = (cp1 - cm1) / (hm1 + h0);
-            Txx = dxx;
-            Tyy = dyy;
-            Tzz = dzz;
-            T_kp1 = cp1;
-        }
-    }
-    const double rho = c.T_inf * c.rho_inf / T;
-    const double lap_u = lap[0], lap_v = lap[1], lap_w = lap[2], lap_T = lap[3];
-    int col = i + Nx * j;
-    double mu = c.mu, kT = c.kT, dk = 0.0;
-    double Y = 0.0;
-    double q_src = wf_heat_source_y(c, col, i, j, k, T, T_kp1, X.Y, &Y);
-    if (c.suth_mu == 1) mu = 1.716e-5 * pow(T / 273.15, 1.5) * (273.15 + 110.4) / (T + 110.4);
-    if (c.suth_k == 1) wf_sutherland_k(T, &kT, &dk);
-    double nu = mu / rho;
-    double T2 = T * T, T3 = T * T * T;
-    double T_RHS_tmp = (dk + 12 * WF_SIGMA_D * c.delta * T2 * (Tx * Tx + Ty * Ty + Tz * Tz) +
-                        (kT + 4 * WF_SIGMA_D * c.delta * T3) * lap_T) / (rho * c.c_p) + q_src;
-    double T_RHS;
-    if (c.temperature_convection == 0)
-        T_RHS = T_RHS_tmp - (u * Tx + v * Ty + w * Tz);
-    else
-        T_RHS = T_RHS_tmp - adv[3];
-    double mod_U = sqrt(u * u + v * v + w * w);
-    double F_x = -0.5 * c.C_d * c.alpha_s * c.sigma_s * Y * mod_U * u;
-    double F_y = -0.5 * c.C_d * c.alpha_s * c.sigma_s * Y * mod_U * v;
-    double F_z = c.g * (rho - c.rho_inf) / rho - 0.5 * c.C_d * c.alpha_s * c.sigma_s * Y * mod_U * w;
-    double u_RHS = nu * lap_u - adv[0] + F_x;
-    double v_RHS = nu * lap_v - adv[1] + F_y;
-    double w_RHS = nu * lap_w - adv[2] + F_z;
-
-    /* SGS divergence (src/cu/turbulence.cu:291-452) */
-    double nu_t = c.mu / rho;
-    double ut = D.utau[col];
-    double ut_ip = D.utau[ip1 + Nx * j], ut_im = D.utau[im1 + Nx * j];
-    double ut_jp = D.utau[i + Nx * jp1], ut_jm = D.utau[i + Nx * jm1];
-    double fw = wf_fw(wf_zibm(c, col, k), ut, nu_t);
-    double fw_ip = wf_fw(wf_zibm(c, ip1 + Nx * j, k), ut_ip, nu_t);
-    double fw_im = wf_fw(wf_zibm(c, im1 + Nx * j, k), ut_im, nu_t);
-    double fw_jp = wf_fw(wf_zibm(c, i + Nx * jp1, k), ut_jp, nu_t);
-    double fw_jm = wf_fw(wf_zibm(c, i + Nx * jm1, k), ut_jm, nu_t);
-    double fw_km = wf_fw(wf_zibm(c, col, k - 1), ut, nu_t);
-    double fw_kp = wf_fw(wf_zibm(c, col, k + 1), ut, nu_t);
-    double mu_c = D.mu[q];
-    double mu_ip = D.mu[qxp1], mu_im = D.mu[qxm1], mu_jp = D.mu[qyp1], mu_jm = D.mu[qym1];
-    double mu_km = D.mu[q - P], mu_kp = D.mu[q + P];
-    mu_c *= fw * fw;
-    mu_im *= fw_im * fw_im;
-    mu_ip *= fw_ip * fw_ip;
-    mu_jm *= fw_jm * fw_jm;
-    mu_jp *= fw_jp * fw_jp;
-    mu_km *= fw_km * fw_km;
-    mu_kp *= fw_kp * fw_kp;
-    double S11 = D.s11[q], S12 = D.s12[q], S13 = D.s13[q], S22 = D.s22[q], S23 = D.s23[q], S33 = D.s33[q];
-    double mu_x = (mu_ip - mu_im) / (2 * c.dx);
-    double mu_y = (mu_jp - mu_jm) / (2 * c.dy);
-    double S11_x = (D.s11[qxp1] - D.s11[qxm1]) / (2.0 * c.dx);
-    double S12_x = (D.s12[qxp1] - D.s12[qxm1]) / (2.0 * c.dx);
-    double S13_x = (D.s13[qxp1] - D.s13[qxm1]) / (2.0 * c.dx);
-    double S21_y = (D.s12[qyp1] - D.s12[qym1]) / (2.0 * c.dy);
-    double S22_y = (D.s22[qyp1] - D.s22[qym1]) / (2.0 * c.dy);
-    double S23_y = (D.s23[qyp1] - D.s23[qym1]) / (2.0 * c.dy);
-    double dU_x = (D.divU[qxp1] - D.divU[qxm1]) / (2.0 * c.dx);
-    double dU_y = (D.divU[qyp1] - D.divU[qym1]) / (2.0 * c.dy);
-    double mu_z = (mu_kp - mu_km) / (h0 + hm1);
-    double S31_z = (D.s13[q + P] - D.s13[q - P]) / (h0 + hm1);
-    double S32_z = (D.s23[q + P] - D.s23[q - P]) / (h0 + hm1);
-    double S33_z = (D.s33[q + P] - D.s33[q - P]) / (h0 + hm1);
-    double dU_z = (D.divU[q + P] - D.divU[q - P]) / (h0 + hm1);
-    double div_S_x = S11_x + S21_y + S31_z;
-    double div_S_y = S12_x + S22_y + S32_z;
-    double div_S_z = S13_x + S23_y + S33_z;
-    double sgs_x = -2 * (mu_x * S11 + mu_y * S12 + mu_z * S13 + mu_c * div_S_x) / rho;
-    double sgs_y = -2 * (mu_x * S12 + mu_y * S22 + mu_z * S23 + mu_c * div_S_y) / rho;
-    double sgs_z = -2 * (mu_x * S13 + mu_y * S23 + mu_z * S33 + mu_c * div_S_z) / rho;
-    double sgs_q = -(mu_x * Tx + mu_y * Ty + mu_z * Tz + mu_c * (Txx + Tyy + Tzz)) / (rho * c.Pr);
-    out[0] = u_RHS + (-sgs_x + dU_x / 3.0);
-    out[1] = v_RHS + (-sgs_y + dU_y / 3.0);
-    out[2] = w_RHS + (-sgs_z + dU_z / 3.0);
-    out[3] = T_RHS - sgs_q;
-}
-
-/* Raw fuel RHS -Y_f K(T) Y where the column stores fuel (src/cu/pde.cu:672-675);
- * zero elsewhere. */
-__device__ __forceinline__ double wf_rhs_fuel(const WfConst &c, const WfState &X, int col, long q, int k, int nzy) {
-    if (k >= nzy) return 0.0;
-    double T = X.T[q];
-    double Y = X.Y[q];
-    return -c.Y_f * wf_rate(c, T) * Y;
+    r[4] = (k < nzy) ? -c.Y_f * K_T * a.Y() : 0.0;
 }
 
 /* RK combination of a node (src/cu/utils.cu:14-21, src/cu/solver.cu:14-39).
@@ -523,10 +487,10 @@ __device__ __forceinline__ void wf_keep_raw(const WfConst &c, const WfDerived &D
 __global__ __launch_bounds__(WF_NT_STAGE) void k_stage(WfConst c, WfState X, WfDerived D, WfState Yn, WfState acc,
                                                        WfState out, int mode, double a) {
     __shared__ WfStageTiles L;
-    const int t = wf_xcd_tile(blockIdx.x, gridDim.x);
     const int nty = (c.Ny + WF_TILE_Y - 1) / WF_TILE_Y;
-    const int k = t % (c.Nz - 1); /* the top plane is completed per column */
-    const int colt = t / (c.Nz - 1);
+    const int ntx = (c.Nx + WF_TILE_X - 1) / WF_TILE_X;
+    int colt, k;
+    wf_tile_decode(ntx * nty, c.Nz - 1, &colt, &k); /* the top plane is completed per column */
     const int j0 = (colt % nty) * WF_TILE_Y, i0 = (colt / nty) * WF_TILE_X;
     const int tx = threadIdx.x, ty = threadIdx.y, tid = tx + WF_TILE_X * ty;
     const int i = i0 + tx, j = j0 + ty;
@@ -606,7 +570,8 @@ __global__ __launch_bounds__(WF_NT_STAGE) void k_stage(WfConst c, WfState X, WfD
         r[3] = c.T_dead;
         r[4] = (k < nzy) ? c.Y_dead : 0.0;
     } else {
-        wf_rhs_tile(c, L, V, tx, ty, col, k, nzy, r);
+        const WfTileAcc acc_t{L, V, (ty + WF_SH) * WF_SW + (tx + WF_SH), (ty + WF_DH) * WF_DW + (tx + WF_DH)};
+        wf_rhs_node(c, acc_t, k, nzy, r);
         wf_keep_raw(c, D, col, k, cut, r);
     }
     wf_rk_write(mode, a, q, r, (k < c.Nzm) ? 5 : 4, Yn, acc, out);
@@ -633,8 +598,8 @@ __global__ __launch_bounds__(256) void k_stage_dup(WfConst c, WfState X, WfDeriv
         r[3] = c.T_dead;
         r[4] = (k < nzy) ? c.Y_dead : 0.0;
     } else {
-        wf_rhs_interior(c, X, D, i, j, k, r);
-        r[4] = (k < nzy) ? -c.Y_f * wf_rate(c, X.T[q]) * X.Y[q] : 0.0;
+        const WfGlobalAcc acc_g{c, X, D, i, j, k};
+        wf_rhs_node(c, acc_g, k, nzy, r);
         wf_keep_raw(c, D, col, k, cut, r);
     }
     wf_rk_write(mode, a, (unsigned)q, r, (k < c.Nzm) ? 5 : 4, Yn, acc, out);
