@@ -61,8 +61,8 @@ def kernel_bytes(params, stats):
     later = d * 3 * N + spec
     out["pressure_rhs_fx"] = (first + (it - 1) * later) / it
     out["pressure_fy"] = 2 * spec
-    # d in, x out, forward multipliers and reciprocal pivots (real) in
-    out["thomas"] = 2 * spec + 2 * 8.0 * hx * ny * nz
+    # d in, x out, the pivot entering each 8-level z-chunk (real) in
+    out["thomas"] = 2 * spec + 8.0 * hx * ny * ((nz + 7) // 8)
     out["pressure_inverse_fx"] = spec + 2 * d * N
     out["project"] = d * (S + N + N + S)
     return out

@@ -10,7 +10,9 @@ import re
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 HEADER = os.path.join(ROOT, "include", "wildfires.h")
-LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libwildfires_amd.so")
+# the in-tree build; WILDFIRES_AMD_LIB names another build of the same library
+LIB_PATH = os.environ.get("WILDFIRES_AMD_LIB") or os.path.join(os.path.dirname(os.path.abspath(__file__)),
+                                                                "libwildfires_amd.so")
 
 _CTYPES = {
     "int": ctypes.c_int,

@@ -1,15 +1,14 @@
 /*
- * wf_fft.h -- mixed-radix Stockham FFT building blocks for lengths with any
- * prime factors (the periodic x,y node counts Nx-1, Ny-1: 63, 127, 255, 510,
- * 511 ...).  Host/device-neutral so the same indexing runs in the LDS kernels
- * and in the CPU check of the plan.
+ * wf_fft.h -- FFT building blocks of the pressure solve.
  *
- * Stage with radix R after radices of product Ns (natural-order Stockham,
- * decimation in time): butterfly j in [0, n/R) reads in[j + r n/R], r < R, and
- * writes out[(j/Ns) Ns R + j%Ns + q Ns], q < R, with
- *   out = sum_r W_{Ns R}^{r (j%Ns + q Ns)} in[j + r n/R],  W_m = exp(sign 2 pi i / m).
- * Each output element is produced independently ("thread per output"), so a
- * pass is a fully parallel loop over n outputs.
+ * Lengths 63 and 255 (the 64- and 256-node grids) use a two-factor register
+ * transform (below).  Any other length runs a mixed-radix Stockham transform
+ * in LDS with run-time radices: stage with radix R after radices of product
+ * Ns (natural order, decimation in time): butterfly j in [0, n/R) reads
+ * in[j + r n/R], r < R, and writes out[(j/Ns) Ns R + j%Ns + q Ns], q < R, with
+ *   out = sum_r W_{Ns R}^{r (j%Ns + q Ns)} in[j + r n/R],  W_m = exp(sign 2 pi i / m),
+ * one output element per thread.  wf_stockham_output is host/device-neutral so
+ * the CPU check of the plan runs the same indexing.
  */
 #ifndef WF_FFT_H
 #define WF_FFT_H
@@ -76,6 +75,18 @@ WF_HD wfc wf_pack_pair(wfc A, wfc B, int conj) {
 }
 
 #if defined(__HIPCC__)
+__device__ __forceinline__ wfc wf_cmul(wfc a, wfc b) {
+#pragma clang fp contract(fast)
+    wfc r;
+    r.re = a.re * b.re - a.im * b.im;
+    r.im = a.re * b.im + a.im * b.re;
+    return r;
+}
+
+/* One Stockham stage of radix R after radices of product NS over `nseq`
+ * sequences of length N (same mathematics as wf_stockham_output).  Radices up
+ * to 8 run one butterfly per thread with its R values in registers; larger
+ * prime radices run one output per thread to keep the workgroup busy. */
 /* ------------------------------------------------------------------------
  * Compile-time plans for the transform lengths of the supported grids.
  * The radix sequence of N is the one the host factoriser produces (4s, then
@@ -105,13 +116,6 @@ constexpr int wf_radix_of(int n, int idx) {
     return 0;
 }
 
-__device__ __forceinline__ wfc wf_cmul(wfc a, wfc b) {
-#pragma clang fp contract(fast)
-    wfc r;
-    r.re = a.re * b.re - a.im * b.im;
-    r.im = a.re * b.im + a.im * b.re;
-    return r;
-}
 
 /* One Stockham stage of radix R after radices of product NS over `nseq`
  * sequences of length N (same mathematics as wf_stockham_output).  Radices up
@@ -176,12 +180,14 @@ __device__ __forceinline__ void wf_stage_ip(wfc *buf, const wfc *tw) {
                 const wfc x0 = src[0];
                 double c0r = x0.re, c0i = x0.im;
                 double cr[QN], ci[QN], sr[QN], si[QN];
+                int m[QN]; /* (r q) mod R for this thread's q = 1 + u + qi QS */
 #pragma unroll
                 for (int qi = 0; qi < QN; qi++) {
                     cr[qi] = x0.re;
                     ci[qi] = x0.im;
                     sr[qi] = 0.0;
                     si[qi] = 0.0;
+                    m[qi] = 0;
                 }
                 constexpr int UNR = (H <= 8) ? H : 4;
 #pragma unroll UNR
@@ -198,14 +204,16 @@ __device__ __forceinline__ void wf_stage_ip(wfc *buf, const wfc *tw) {
 #pragma unroll
                     for (int qi = 0; qi < QN; qi++) {
                         const int q = 1 + u + qi * QS;
-                        const int m = (r * q) % R;
-                        const int mm = (m <= H) ? m : R - m;
+                        int mm = m[qi] + q;
+                        mm = (mm >= R) ? mm - R : mm;
+                        m[qi] = mm;
+                        /* tw[m N/R] = W^m (its imaginary part carries the sign
+                         * of W^{R-m} = conj W^m) */
                         const wfc W = tw[mm * WR];
-                        const double sw = (m <= H) ? W.im : -W.im;
                         cr[qi] += ar * W.re;
                         ci[qi] += ai * W.re;
-                        sr[qi] += bi * sw;
-                        si[qi] += br * sw;
+                        sr[qi] += bi * W.im;
+                        si[qi] += br * W.im;
                     }
                 }
                 y0[pi].re = c0r;
@@ -293,6 +301,103 @@ struct WfStagesIP {
 template <int N, int NSEQ, int NT>
 __device__ __forceinline__ void wf_fft_ip(wfc *buf, const wfc *tw) {
     WfStagesIP<N, 0, 1, NSEQ, NT>::run(buf, tw);
+}
+/* ------------------------------------------------------------------------
+ * Two-factor register FFT for N = N1 N2 (N1, N2 odd, <= 17): the transform
+ * lengths of the 64- and 256-node grids (63 = 7 x 9, 255 = 15 x 17).
+ *   n = n1 + N1 n2, k = k2 + N2 k1:
+ *   Y[n1][k2] = sum_{n2} x[n1 + N1 n2] W_{N2}^{n2 k2}          (phase A)
+ *   X[k2 + N2 k1] = sum_{n1} W_N^{n1 k2} Y[n1][k2] W_{N1}^{n1 k1} (phase B)
+ * Phase A: one thread per (sequence, n1) loads its N2 inputs, transforms them
+ * in registers, twiddles and writes Y to LDS at n1 + N1 k2 (the positions it
+ * read, when the input is that LDS buffer).  Phase B: one thread per
+ * (sequence, k2) reads its N1 contiguous values of Y, transforms them and
+ * hands X[k2 + N2 k1] to the caller.  The DFT constants W_R^m = tw[m N/R]
+ * have compile-time offsets into the N-point table (scalar loads).
+ * ---------------------------------------------------------------------- */
+
+/* Factor pair of the lengths with a two-factor plan (0 if none). */
+__host__ __device__ constexpr int wf_f2_n1(int n) { return n == 255 ? 15 : n == 63 ? 7 : 0; }
+__host__ __device__ constexpr int wf_f2_n2(int n) { return n == 255 ? 17 : n == 63 ? 9 : 0; }
+/* sequences per 256-thread workgroup */
+__host__ __device__ constexpr int wf_f2_nseq(int n) {
+    return wf_f2_n1(n) ? 256 / (wf_f2_n1(n) > wf_f2_n2(n) ? wf_f2_n1(n) : wf_f2_n2(n)) : 0;
+}
+
+/* Odd R-point DFT of x (registers) with W = exp(SIGN 2 pi i / R) taken from
+ * the forward N-point table tw (W_R^m = tw[m TS] or its conjugate), outputs
+ * handed to out(k, X_k) in the order 0, 1, R-1, 2, R-2, ... (pairs q, R-q
+ * share their sums: X_q, X_{R-q} = x_0 + sum a_r Re W^{rq} +- i sum b_r Im W^{rq}). */
+template <int R, int TS, int SIGN, class Out>
+__device__ __forceinline__ void wf_dft_reg(const wfc *x, const double2 *tw, Out &&out) {
+#pragma clang fp contract(fast)
+    constexpr int H = (R - 1) / 2;
+    wfc a[H + 1], b[H + 1];
+    wfc y0 = x[0];
+#pragma unroll
+    for (int r = 1; r <= H; r++) {
+        a[r].re = x[r].re + x[R - r].re;
+        a[r].im = x[r].im + x[R - r].im;
+        b[r].re = x[r].re - x[R - r].re;
+        b[r].im = x[r].im - x[R - r].im;
+        y0.re += a[r].re;
+        y0.im += a[r].im;
+    }
+    out(0, y0);
+#pragma unroll
+    for (int q = 1; q <= H; q++) {
+        double cr = x[0].re, ci = x[0].im, sr = 0.0, si = 0.0;
+#pragma unroll
+        for (int r = 1; r <= H; r++) {
+            const double2 W = tw[((r * q) % R) * TS];
+            const double c = W.x, sn = (SIGN < 0) ? W.y : -W.y;
+            cr += a[r].re * c;
+            ci += a[r].im * c;
+            sr += b[r].im * sn;
+            si += b[r].re * sn;
+        }
+        wfc yp, ym;
+        yp.re = cr - sr;
+        yp.im = ci + si;
+        ym.re = cr + sr;
+        ym.im = ci - si;
+        out(q, yp);
+        out(R - q, ym);
+    }
+}
+
+/* Phase A for sequence s, n1 = this thread's row: load(s, n) gives input n;
+ * Y is written to buf[s N + n1 + N1 k2]. */
+template <int N1, int N2, int SIGN, class Load>
+__device__ __forceinline__ void wf_f2_phase_a(wfc *buf, const double2 *tw, int s, int n1, Load &&load) {
+#pragma clang fp contract(fast)
+    constexpr int N = N1 * N2;
+    wfc x[N2];
+#pragma unroll
+    for (int n2 = 0; n2 < N2; n2++) x[n2] = load(s, n1 + N1 * n2);
+    wfc *row = buf + s * N + n1;
+    wf_dft_reg<N2, N1, SIGN>(x, tw, [&](int k2, wfc y) {
+        if (k2 != 0) { /* twiddle W_N^{n1 k2}; n1 k2 < N */
+            const double2 W = tw[n1 * k2];
+            const double wi = (SIGN < 0) ? W.y : -W.y;
+            wfc t;
+            t.re = y.re * W.x - y.im * wi;
+            t.im = y.re * wi + y.im * W.x;
+            y = t;
+        }
+        row[N1 * k2] = y;
+    });
+}
+
+/* Phase B for sequence s, k2 = this thread's column: X[k2 + N2 k1] -> out(k1, X). */
+template <int N1, int N2, int SIGN, class Out>
+__device__ __forceinline__ void wf_f2_phase_b(const wfc *buf, const double2 *tw, int s, int k2, Out &&out) {
+    constexpr int N = N1 * N2;
+    wfc y[N1];
+    const wfc *col = buf + s * N + N1 * k2;
+#pragma unroll
+    for (int n1 = 0; n1 < N1; n1++) y[n1] = col[n1];
+    wf_dft_reg<N1, N2, SIGN>(y, tw, out);
 }
 #endif
 

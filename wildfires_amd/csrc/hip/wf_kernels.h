@@ -96,8 +96,8 @@ struct WfSpectral {
     double2 *F;     /* [nz][hx][ldj] spectral working buffer (ky fastest) */
     double2 *Ptop;  /* [hx][ldj] spectrum of the top pressure plane */
     double *A;      /* Nx*Ny*Nz: p-independent part of the Poisson RHS */
-    const double *L;     /* [nz][hx][ldj]: forward multipliers a_{k-1}/u_{k-1} (k >= 1) */
-    const double *RU;    /* [nz][hx][ldj]: 1/u_k, u_k the Thomas pivots */
+    const double *UB;    /* [chunk][hx][ldj]: Thomas pivot u_{k0-1} entering each z-chunk (chunk >= 1) */
+    int thomas_lc;       /* z-levels per chunk of the tridiagonal solve */
     double *pmax;        /* npmax partial maxima of |p_new - p_old| */
     const double *kx2;   /* hx: kx^2 of the retained x-wavenumbers */
     const double *ky2;   /* ny */
@@ -113,6 +113,7 @@ struct WfSpectral {
     const double *beta;            /* nz: 2/(dz_{k-1} dz_k) for 1 <= k <= nz-2 */
     const double *rho;             /* rho^n on every node (variable density) */
     WfIterCtl *ctl;
+    int *hflag;                    /* mapped host memory: [0] done, [1] solves finished */
 };
 
 /* Forms of the x-transform input: the Poisson RHS of a later iteration (stored
@@ -126,10 +127,10 @@ hipError_t wf_launch_fy(const WfSpectral &sp, double2 *buf, long slab_stride, in
                         int guarded, hipStream_t s);
 hipError_t wf_launch_density(const WfConst &c, const double *T, double *rho, hipStream_t s);
 hipError_t wf_launch_iter_reset(WfIterCtl *ctl, hipStream_t s);
-hipError_t wf_launch_iter_check(const WfSpectral &sp, double tol, int m, int max_iter, hipStream_t s);
 hipError_t wf_launch_thomas(const WfSpectral &sp, hipStream_t s);
-hipError_t wf_launch_pivots(const WfSpectral &sp, double *L, double *RU, hipStream_t s);
+hipError_t wf_launch_pivots(const WfSpectral &sp, double *UB, hipStream_t s);
 hipError_t wf_launch_inverse_fx(const WfConst &c, const WfSpectral &sp, double *p, hipStream_t s);
+hipError_t wf_launch_iter_check(const WfSpectral &sp, double tol, int m, int max_iter, hipStream_t s);
 
 /* Temperature source applied at the end of a step (src/cu/functions.cu:325-403). */
 struct WfSource {

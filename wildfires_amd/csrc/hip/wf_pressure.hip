@@ -10,9 +10,10 @@
  * nx/2+1 retained kx bins only; the discarded half is the complex conjugate.
  *
  *   P1 k_rhs_fx   : f (or the top pressure plane) -> x half-spectrum   [row-pair group, slab]
- *   P2 k_fy       : y-transform of WF_FY_COLS kx columns, one slab      [forward and inverse]
+ *   P2 k_fy       : y-transform of sp.fy_cols kx columns, one slab      [forward and inverse]
  *   P3 k_thomas   : tridiagonal solve, 64 (kx, ky) systems per workgroup, z split over its waves
  *   P5 k_inv_fx   : inverse x-transform, p, periodic duplicates, max |dp| per workgroup
+ *   k_iter_check  : convergence test of the solve
  *
  * Spectral layout: F[k][kx][ky] with ky fastest (column stride ldj), so a
  * y-transform reads whole contiguous columns and the tridiagonal solve reads
@@ -27,6 +28,12 @@
 #include "wf_kernels.h"
 
 #define WF_NT 256
+
+/* 32-bit element offsets from a uniform base (scalar base + 32-bit lane offset;
+ * the solver limits Nx*Ny*Nz to 2^28). */
+__device__ __forceinline__ double wf_ldp(const double *base, unsigned q) {
+    return *(const double *)((const char *)base + (q << 3));
+}
 
 /* One full FFT of `nseq` sequences of run-time length n held in LDS
  * (a -> b -> a ...), for lengths without a compile-time plan.  Returns the
@@ -130,22 +137,27 @@ __device__ double wf_f_fixed(const WfConst &c, const WfState &Ys, const WfState 
 }
 
 /* (grad rho . grad p) / rho at node (i,j,k), 1 <= k <= Nz-2, with rho = rho^n
- * the density of y_n (src/cu/pressure.cu:163-197, 269-282, 319). */
+ * the density of y_n (src/cu/pressure.cu:163-197, 269-282, 319).  The
+ * constant divisors 2dx, 2dy, z[k+1]-z[k-1] are taken as reciprocals and the
+ * sum divided by rho once. */
 __device__ __forceinline__ double wf_f_pgrad(const WfConst &c, const double *rho_n, const double *p, int i, int j,
                                              int k) {
+#pragma clang fp contract(fast)
     const int Nx = c.Nx, Ny = c.Ny;
-    const long P = c.plane;
-    const double *z = c.z;
-    int im1 = wf_wrap_m(i, 1, Nx), ip1 = wf_wrap_p(i, 1, Nx);
-    int jm1 = wf_wrap_m(j, 1, Ny), jp1 = wf_wrap_p(j, 1, Ny);
-    long q = wf_q(c, i, j, k);
-    long qxm = wf_q(c, im1, j, k), qxp = wf_q(c, ip1, j, k), qym = wf_q(c, i, jm1, k), qyp = wf_q(c, i, jp1, k);
-    double dz_km1 = z[k] - z[k - 1], dz_k = z[k + 1] - z[k];
-    double rho = rho_n[q];
-    double rhox = (rho_n[qxp] - rho_n[qxm]) / (2 * c.dx), rhoy = (rho_n[qyp] - rho_n[qym]) / (2 * c.dy);
-    double px = (p[qxp] - p[qxm]) / (2 * c.dx), py = (p[qyp] - p[qym]) / (2 * c.dy);
-    double rhoz = (rho_n[q + P] - rho_n[q - P]) / (dz_k + dz_km1);
-    double pz = (p[q + P] - p[q - P]) / (dz_k + dz_km1);
+    const unsigned P = (unsigned)c.plane;
+    const int im1 = wf_wrap_m(i, 1, Nx), ip1 = wf_wrap_p(i, 1, Nx);
+    const int jm1 = wf_wrap_m(j, 1, Ny), jp1 = wf_wrap_p(j, 1, Ny);
+    const unsigned q = (unsigned)wf_q(c, i, j, k);
+    const unsigned qxm = (unsigned)wf_q(c, im1, j, k), qxp = (unsigned)wf_q(c, ip1, j, k);
+    const unsigned qym = (unsigned)wf_q(c, i, jm1, k), qyp = (unsigned)wf_q(c, i, jp1, k);
+    const double dzc = c.zc[WF_ZC * k + WF_Z_DZC];
+    const double rho = wf_ldp(rho_n, q);
+    const double rhox = (wf_ldp(rho_n, qxp) - wf_ldp(rho_n, qxm)) * c.inv2dx;
+    const double rhoy = (wf_ldp(rho_n, qyp) - wf_ldp(rho_n, qym)) * c.inv2dy;
+    const double px = (wf_ldp(p, qxp) - wf_ldp(p, qxm)) * c.inv2dx;
+    const double py = (wf_ldp(p, qyp) - wf_ldp(p, qym)) * c.inv2dy;
+    const double rhoz = (wf_ldp(rho_n, q + P) - wf_ldp(rho_n, q - P)) * dzc;
+    const double pz = (wf_ldp(p, q + P) - wf_ldp(p, q - P)) * dzc;
     return (rhox * px + rhoy * py + rhoz * pz) / rho;
 }
 
@@ -162,28 +174,38 @@ __global__ void k_iter_reset(WfIterCtl *ctl) {
     ctl->err = 0.0;
 }
 
-/* After solve m: err = max|p_new - p_old| (maximum of the inverse transform's
- * per-workgroup maxima), stop once err <= tol or after max_iter solves
- * (src/cu/pressure.cu:564-574); iters is the count the reference reports (the
- * index of the converging solve, else max_iter). */
-__global__ __launch_bounds__(WF_NT) void k_iter_check(WfIterCtl *ctl, const double *pmax, int npmax, double tol,
-                                                      int m, int max_iter) {
+/* After solve m: err = max|p_new - p_old| (the maximum of the inverse
+ * x-transform's per-workgroup maxima), stop once err <= tol or after max_iter
+ * solves (src/cu/pressure.cu:564-574); iters is the count the reference
+ * reports (the index of the converging solve, else max_iter).  The flag and
+ * the solve count are also written to mapped host memory so the host can stop
+ * queueing solves without a round trip.  (A separate one-workgroup launch: on
+ * a multi-XCD device a grid-wide reduction inside the inverse transform would
+ * need a device-scope release, an L2 write-back, in every workgroup.) */
+__global__ __launch_bounds__(WF_NT) void k_iter_check(WfSpectral sp, double tol, int m, int max_iter) {
+    WfIterCtl *ctl = sp.ctl;
     if (ctl->done) return;
     double v = 0.0;
-    for (int t = threadIdx.x; t < npmax; t += blockDim.x) v = (pmax[t] > v) ? pmax[t] : v;
+    for (int t = threadIdx.x; t < sp.npmax; t += blockDim.x) v = (sp.pmax[t] > v) ? sp.pmax[t] : v;
     __shared__ double err;
     wf_block_max_store(v, &err);
     __syncthreads();
     if (threadIdx.x != 0) return;
     ctl->err = err;
     ctl->solves = m + 1;
+    int done;
     if (err <= tol) {
-        ctl->done = 1;
+        done = 1;
         ctl->iters = m;
     } else {
         ctl->iters = m + 1;
-        if (m + 1 >= max_iter) ctl->done = 1;
+        done = (m + 1 >= max_iter) ? 1 : 0;
     }
+    ctl->done = done;
+    volatile int *hf = sp.hflag;
+    hf[1] = m + 1;
+    hf[0] = done;
+    __threadfence_system();
 }
 
 /* Input value of the x-transform at (i, j, k) for form MODE. */
@@ -266,46 +288,59 @@ __global__ __launch_bounds__(WF_NT) void k_rhs_fx(WfConst c, WfSpectral sp, WfSt
     }
 }
 
-/* P2/P4: y-transform of WF_FY_COLS kx columns (each contiguous over ky) of
- * slab slab0 + blockIdx.y. */
+/* P2/P4: y-transform of sp.fy_cols kx columns (each contiguous over ky) of
+ * slab slab0 + blockIdx.y; sign -1 forward, +1 inverse. */
+template <int NY, int SIGN>
+__device__ __forceinline__ void wf_fy_f2(const WfSpectral &sp, double2 *base, wfc *buf, int r0) {
+    constexpr int N1 = wf_f2_n1(NY), N2 = wf_f2_n2(NY), NSEQ = wf_f2_nseq(NY);
+    const int hx = sp.hx, ldj = sp.ldj;
+    const int t = threadIdx.x;
+    if (t < N1 * NSEQ) {
+        const int s = t / N1, n1 = t - s * N1;
+        wf_f2_phase_a<N1, N2, SIGN>(buf, sp.twy, s, n1, [&](int ss, int n) {
+            wfc v = {0.0, 0.0};
+            if (r0 + ss < hx) {
+                const double2 x = base[(long)(r0 + ss) * ldj + n];
+                v.re = x.x;
+                v.im = x.y;
+            }
+            return v;
+        });
+    }
+    __syncthreads();
+    if (t < N2 * NSEQ) {
+        const int s = t / N2, k2 = t - s * N2;
+        if (r0 + s < hx) {
+            double2 *col = base + (long)(r0 + s) * ldj + k2;
+            wf_f2_phase_b<N1, N2, SIGN>(buf, sp.twy, s, k2, [&](int k1, wfc X) { col[N2 * k1] = make_double2(X.re, X.im); });
+        }
+    }
+}
+
 template <int NY>
 __global__ __launch_bounds__(WF_NT) void k_fy(WfSpectral sp, double2 *buf, long slab_stride, int slab0, int sign,
                                               int guarded) {
     if (guarded && sp.ctl->done) return;
     extern __shared__ wfc smem[];
-    __shared__ int rad[WF_MAX_RADICES];
-    const int ny = (NY > 0) ? NY : sp.ny;
-    const int hx = sp.hx, ldj = sp.ldj;
-    constexpr int COLS = WF_FY_COLS;
-    wfc *tw = smem, *bufa = smem + ny, *bufb = bufa + COLS * ny;
-    const int r0 = blockIdx.x * COLS;
     double2 *base = buf + (long)(slab0 + blockIdx.y) * slab_stride;
-    for (int t = threadIdx.x; t < ny; t += blockDim.x) {
-        double2 w = sp.twy[t];
-        tw[t].re = w.x;
-        tw[t].im = (sign > 0) ? -w.y : w.y;
-    }
-    if (NY == 0 && threadIdx.x < WF_MAX_RADICES) rad[threadIdx.x] = sp.rady[threadIdx.x];
+    const int cols = sp.fy_cols;
+    const int r0 = blockIdx.x * cols;
     if constexpr (NY > 0) {
-        constexpr int PER = (COLS * NY + WF_NT - 1) / WF_NT;
-        double2 v[PER];
-#pragma unroll
-        for (int pi = 0; pi < PER; pi++) {
-            const int t = threadIdx.x + pi * WF_NT;
-            const int cc = t / NY, jj = t - cc * NY;
-            v[pi] = make_double2(0.0, 0.0);
-            if (t < COLS * NY && r0 + cc < hx) v[pi] = base[(long)(r0 + cc) * ldj + jj];
-        }
-#pragma unroll
-        for (int pi = 0; pi < PER; pi++) {
-            const int t = threadIdx.x + pi * WF_NT;
-            if (t < COLS * NY) {
-                bufa[t].re = v[pi].x;
-                bufa[t].im = v[pi].y;
-            }
-        }
+        if (sign < 0)
+            wf_fy_f2<NY, -1>(sp, base, smem, r0);
+        else
+            wf_fy_f2<NY, 1>(sp, base, smem, r0);
     } else {
-        for (int t = threadIdx.x; t < COLS * ny; t += blockDim.x) {
+        __shared__ int rad[WF_MAX_RADICES];
+        const int ny = sp.ny, hx = sp.hx, ldj = sp.ldj;
+        wfc *tw = smem, *bufa = smem + ny, *bufb = bufa + cols * ny;
+        for (int t = threadIdx.x; t < ny; t += blockDim.x) {
+            double2 w = sp.twy[t];
+            tw[t].re = w.x;
+            tw[t].im = (sign > 0) ? -w.y : w.y;
+        }
+        if (threadIdx.x < WF_MAX_RADICES) rad[threadIdx.x] = sp.rady[threadIdx.x];
+        for (int t = threadIdx.x; t < cols * ny; t += blockDim.x) {
             const int cc = t / ny, jj = t - cc * ny;
             wfc v = {0.0, 0.0};
             if (r0 + cc < hx) {
@@ -315,38 +350,44 @@ __global__ __launch_bounds__(WF_NT) void k_fy(WfSpectral sp, double2 *buf, long 
             }
             bufa[t] = v;
         }
-    }
-    __syncthreads();
-    wfc *res = wf_transform<NY, COLS>(bufa, bufb, ny, COLS, rad, tw);
-    for (int t = threadIdx.x; t < COLS * ny; t += blockDim.x) {
-        const int cc = t / ny, jj = t - cc * ny;
-        if (r0 + cc < hx) base[(long)(r0 + cc) * ldj + jj] = make_double2(res[t].re, res[t].im);
+        __syncthreads();
+        wfc *res = wf_lds_fft(bufa, bufb, ny, cols, rad, tw);
+        for (int t = threadIdx.x; t < cols * ny; t += blockDim.x) {
+            const int cc = t / ny, jj = t - cc * ny;
+            if (r0 + cc < hx) base[(long)(r0 + cc) * ldj + jj] = make_double2(res[t].re, res[t].im);
+        }
     }
 }
 
 /* Thomas pivots of every (kx, ky) system (src/cu/pressure.cu:429-433):
  * u_0 = b_0, u_k = b_k - (a_{k-1}/u_{k-1}) c_{k-1}.  They do not change between
- * steps, so they are formed once, stored as the forward multipliers
- * L_k = a_{k-1}/u_{k-1} and the reciprocals 1/u_k. */
-__global__ void k_pivots(WfSpectral sp, double *L, double *RU) {
+ * steps; the pivot entering each z-chunk of the tridiagonal solve (u_{k0-1},
+ * k0 = chunk * lc) is stored once, and the solve re-forms the others with the
+ * same operations. */
+__device__ __forceinline__ double wf_pivot_next(const WfSpectral &sp, double u, int k, double K2, double c0,
+                                                double *l_out) {
+    const int nz = sp.nz;
+    const double ck1 = (k == 1) ? c0 : sp.tc[k - 1];
+    const double bk = (k == nz - 1) ? -(K2 + sp.beta_top) : -(K2 + sp.beta[k]);
+    const double l = sp.ta[k - 1] / u;
+    *l_out = l;
+    return bk - l * ck1;
+}
+
+__global__ void k_pivots(WfSpectral sp, double *UB) {
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
     const int r = blockIdx.y;
     if (j >= sp.ny) return;
-    const int nz = sp.nz;
+    const int nz = sp.nz, lc = sp.thomas_lc;
     const long slab = (long)sp.hx * sp.ldj;
     const long o = (long)r * sp.ldj + j;
-    double K2 = sp.kx2[r] + sp.ky2[j];
-    double c0 = sp.c0a - 0.5 * sp.dz0 * K2 - sp.c0b;
+    const double K2 = sp.kx2[r] + sp.ky2[j];
+    const double c0 = sp.c0a - 0.5 * sp.dz0 * K2 - sp.c0b;
     double u = sp.b0;
-    L[o] = 0.0;
-    RU[o] = 1.0 / u;
     for (int k = 1; k < nz; k++) {
-        double ck1 = (k == 1) ? c0 : sp.tc[k - 1];
-        double bk = (k == nz - 1) ? -(K2 + sp.beta_top) : -(K2 + sp.beta[k]);
-        double l = sp.ta[k - 1] / u;
-        u = bk - l * ck1;
-        L[o + k * slab] = l;
-        RU[o + k * slab] = 1.0 / u;
+        if (k % lc == 0) UB[o + (k / lc) * slab] = u; /* u_{k-1} enters chunk k/lc */
+        double l;
+        u = wf_pivot_next(sp, u, k, K2, c0, &l);
     }
 }
 
@@ -373,8 +414,11 @@ __global__ __launch_bounds__(64 * WF_THOMAS_MAX_CHUNKS) void k_thomas(WfSpectral
     const long slab = (long)sp.hx * sp.ldj;
     const long o = (long)r * sp.ldj + (ok ? j : 0);
     const int k0 = w * LC;
+    const double K2 = ok ? sp.kx2[r] + sp.ky2[j] : 0.0;
+    const double c0 = sp.c0a - 0.5 * sp.dz0 * K2 - sp.c0b;
     double2 y[LC];
     double g[LC], ru[LC];
+    double u = (w > 0 && ok) ? sp.UB[o + w * slab] : sp.b0; /* pivot entering the chunk */
 #pragma unroll
     for (int t = 0; t < LC; t++) {
         const int k = k0 + t;
@@ -385,6 +429,7 @@ __global__ __launch_bounds__(64 * WF_THOMAS_MAX_CHUNKS) void k_thomas(WfSpectral
             if (k == 0) {
                 double2 f1 = sp.F[o + slab];
                 y[t] = make_double2(0.5 * sp.dz0 * f1.x, 0.5 * sp.dz0 * f1.y);
+                u = sp.b0;
             } else {
                 double2 d = sp.F[o + k * slab];
                 if (k == nz - 1) {
@@ -393,9 +438,11 @@ __global__ __launch_bounds__(64 * WF_THOMAS_MAX_CHUNKS) void k_thomas(WfSpectral
                     d.y = d.y - pt.y / sp.top_div;
                 }
                 y[t] = d;
-                g[t] = sp.L[o + k * slab];
+                double l;
+                u = wf_pivot_next(sp, u, k, K2, c0, &l);
+                g[t] = l;
             }
-            ru[t] = sp.RU[o + k * slab];
+            ru[t] = 1.0 / u;
         }
     }
     /* forward elimination, entry value 0, coefficient G of the entry value */
@@ -433,8 +480,6 @@ __global__ __launch_bounds__(64 * WF_THOMAS_MAX_CHUNKS) void k_thomas(WfSpectral
     }
     __syncthreads();
     /* back substitution, entry value x_{k0+LC} = 0, coefficient H */
-    const double K2 = ok ? sp.kx2[r] + sp.ky2[j] : 0.0;
-    const double c0 = sp.c0a - 0.5 * sp.dz0 * K2 - sp.c0b;
     double2 xa = make_double2(0.0, 0.0);
     double H = 1.0;
 #pragma unroll
@@ -473,19 +518,47 @@ __global__ __launch_bounds__(64 * WF_THOMAS_MAX_CHUNKS) void k_thomas(WfSpectral
 }
 
 /* P5: inverse x-transform of one row-pair group of slab k (0..nz-1):
- * p = Re(.)/(nx ny) on unique nodes, duplicate planes i = Nx-1 and j = Ny-1 set
- * from their images [Q6], and max |p_new - p_old| of the workgroup into
- * sp.pmax[blockIdx.x].  The last workgroup refreshes the periodic copies on the
- * top plane, which the solve never writes on unique nodes. */
+ * p = Re(.)/(nx ny) (multiplied by the reciprocal) on unique nodes, duplicate
+ * planes i = Nx-1 and j = Ny-1 set from their images [Q6], and max
+ * |p_new - p_old| of the workgroup into sp.pmax[blockIdx.x].  The last
+ * workgroup refreshes the periodic copies on the top plane, which the solve
+ * never writes on unique nodes. */
+
+/* new value pv of node (i, j, k) and of its periodic duplicates, with the
+ * running max |p_new - p_old| */
+__device__ __forceinline__ void wf_put_p(const WfConst &c, double *p, int i, int j, int k, double pv, double &dmax) {
+    const long q = wf_q(c, i, j, k);
+    const double d = fabs(pv - p[q]);
+    dmax = (d > dmax) ? d : dmax;
+    p[q] = pv;
+    if (i == 0) { /* duplicate column i = Nx-1 */
+        const long qd = wf_q(c, c.Nx - 1, j, k);
+        const double dd = fabs(pv - p[qd]);
+        dmax = (dd > dmax) ? dd : dmax;
+        p[qd] = pv;
+    }
+    if (j == 0) { /* duplicate row j = Ny-1 */
+        const long qd = wf_q(c, i, c.Ny - 1, k);
+        const double dd = fabs(pv - p[qd]);
+        dmax = (dd > dmax) ? dd : dmax;
+        p[qd] = pv;
+        if (i == 0) {
+            const long qc = wf_q(c, c.Nx - 1, c.Ny - 1, k);
+            const double dc = fabs(pv - p[qc]);
+            dmax = (dc > dmax) ? dc : dmax;
+            p[qc] = pv;
+        }
+    }
+}
+
 template <int NX>
 __global__ __launch_bounds__(WF_NT) void k_inv_fx(WfConst c, WfSpectral sp, double *p) {
     if (sp.ctl->done) return;
     extern __shared__ wfc smem[];
     __shared__ int rad[WF_MAX_RADICES];
-    constexpr int PAIRS = (NX > 0) ? wf_pairs_for(NX) : 0;
     const int nx = (NX > 0) ? NX : sp.nx;
-    const int pairs = (NX > 0) ? PAIRS : sp.fx_pairs;
-    const int ny = sp.ny, hx = sp.hx, ldj = sp.ldj, Nx = c.Nx;
+    const int pairs = sp.fx_pairs;
+    const int ny = sp.ny, hx = sp.hx, ldj = sp.ldj;
     double dmax = 0.0;
     if (blockIdx.x == gridDim.x - 1) {
         const int kt = c.Nz - 1;
@@ -506,62 +579,49 @@ __global__ __launch_bounds__(WF_NT) void k_inv_fx(WfConst c, WfSpectral sp, doub
         wf_block_max_store(dmax, &sp.pmax[blockIdx.x]);
         return;
     }
-    wfc *tw = smem, *bufa = smem + nx, *bufb = bufa + pairs * nx;
-    const int tile = wf_xcd_tile(blockIdx.x, gridDim.x - 1);
-    const int g = tile % sp.ngroups, k = tile / sp.ngroups;
+    int g, k;
+    wf_tile_decode(sp.ngroups, sp.nz, &g, &k);
     const int j0 = 2 * g * pairs;
     const double2 *src = sp.F + (long)k * hx * ldj;
-    for (int t = threadIdx.x; t < nx; t += blockDim.x) {
-        double2 w = sp.twx[t];
-        tw[t].re = w.x;
-        tw[t].im = -w.y;
-    }
-    if (NX == 0 && threadIdx.x < WF_MAX_RADICES) rad[threadIdx.x] = sp.radx[threadIdx.x];
-    /* thread t -> pair s = t % pairs, bin r = t / pairs: consecutive threads read
-     * consecutive ky of one kx column */
-    for (int t = threadIdx.x; t < pairs * nx; t += blockDim.x) {
-        const int s = t % pairs, r = t / pairs;
-        const int ja = j0 + 2 * s, jb = ja + 1;
-        wfc A = {0.0, 0.0}, B = {0.0, 0.0};
-        const int conj = r >= hx;
-        const int rr = conj ? nx - r : r;
-        if (ja < ny) {
-            double2 x = src[(long)rr * ldj + ja];
-            A.re = x.x;
-            A.im = x.y;
-            if (jb < ny) {
-                double2 y = src[(long)rr * ldj + jb];
-                B.re = y.x;
-                B.im = y.y;
+    const double inv_scale = 1.0 / ((double)nx * (double)ny);
+    {
+        constexpr int PAIRS = (NX > 0) ? wf_pairs_for(NX) : 0;
+        const int Nx = c.Nx;
+        wfc *tw = smem, *bufa = smem + nx, *bufb = bufa + pairs * nx;
+        for (int t = threadIdx.x; t < nx; t += blockDim.x) {
+            double2 w = sp.twx[t];
+            tw[t].re = w.x;
+            tw[t].im = -w.y;
+        }
+        if (NX == 0 && threadIdx.x < WF_MAX_RADICES) rad[threadIdx.x] = sp.radx[threadIdx.x];
+        /* thread t -> pair s = t % pairs, bin r = t / pairs: consecutive threads
+         * read consecutive ky of one kx column */
+        for (int t = threadIdx.x; t < pairs * nx; t += blockDim.x) {
+            const int s = t % pairs, r = t / pairs;
+            const int ja = j0 + 2 * s, jb = ja + 1;
+            wfc A = {0.0, 0.0}, B = {0.0, 0.0};
+            const int conj = r >= hx;
+            const int rr = conj ? nx - r : r;
+            if (ja < ny) {
+                double2 x = src[(long)rr * ldj + ja];
+                A.re = x.x;
+                A.im = x.y;
+                if (jb < ny) {
+                    double2 y = src[(long)rr * ldj + jb];
+                    B.re = y.x;
+                    B.im = y.y;
+                }
             }
+            bufa[s * nx + r] = wf_pack_pair(A, B, conj);
         }
-        bufa[s * nx + r] = wf_pack_pair(A, B, conj);
-    }
-    __syncthreads();
-    wfc *res = wf_transform<NX, PAIRS>(bufa, bufb, nx, pairs, rad, tw);
-    const double scale = (double)nx * (double)ny;
-    for (int t = threadIdx.x; t < pairs * Nx; t += blockDim.x) {
-        const int s = t / Nx, i = t - s * Nx;
-        const int ja = j0 + 2 * s, jb = ja + 1;
-        if (ja >= ny) continue;
-        const int ii = (i == Nx - 1) ? 0 : i;
-        const double pa = res[s * nx + ii].re / scale;
-        const long qa = wf_q(c, i, ja, k);
-        const double da = fabs(pa - p[qa]);
-        dmax = (da > dmax) ? da : dmax;
-        p[qa] = pa;
-        if (ja == 0) { /* duplicate row j = Ny-1 */
-            const long qd = wf_q(c, i, c.Ny - 1, k);
-            const double dd = fabs(pa - p[qd]);
-            dmax = (dd > dmax) ? dd : dmax;
-            p[qd] = pa;
-        }
-        if (jb < ny) {
-            const double pb = res[s * nx + ii].im / scale;
-            const long qb = wf_q(c, i, jb, k);
-            const double db = fabs(pb - p[qb]);
-            dmax = (db > dmax) ? db : dmax;
-            p[qb] = pb;
+        __syncthreads();
+        wfc *res = wf_transform<NX, PAIRS>(bufa, bufb, nx, pairs, rad, tw);
+        for (int t = threadIdx.x; t < pairs * (Nx - 1); t += blockDim.x) {
+            const int s = t / (Nx - 1), i = t - s * (Nx - 1);
+            const int ja = j0 + 2 * s, jb = ja + 1;
+            if (ja >= ny) continue;
+            wf_put_p(c, p, i, ja, k, res[s * nx + i].re * inv_scale, dmax);
+            if (jb < ny) wf_put_p(c, p, i, jb, k, res[s * nx + i].im * inv_scale, dmax);
         }
     }
     wf_block_max_store(dmax, &sp.pmax[blockIdx.x]);
@@ -569,13 +629,20 @@ __global__ __launch_bounds__(WF_NT) void k_inv_fx(WfConst c, WfSpectral sp, doub
 
 /* ------------------------------------------------------------- launchers ---- */
 
-/* lengths with a compile-time plan */
+/* lengths with a compile-time Stockham plan (x-transforms) */
 static bool wf_planned(int n) { return n == 63 || n == 127 || n == 255 || n == 511; }
 
-/* dynamic LDS of an x/y transform kernel: twiddles + one buffer (compile-time
+/* dynamic LDS of an x-transform kernel: twiddles + one buffer (compile-time
  * plan, in place) or two buffers (run-time radices) */
-static size_t wf_fft_lds(int n, int nseq) {
+static size_t wf_fx_lds(int n, int nseq) {
     return sizeof(wfc) * (size_t)n * (1 + (wf_planned(n) ? 1 : 2) * (size_t)nseq);
+}
+
+/* dynamic LDS of the y-transform: the two-factor plans use one buffer of nseq
+ * sequences; run-time radices the twiddle table and two buffers */
+static size_t wf_fy_lds(int n, int nseq) {
+    if (wf_f2_n1(n)) return sizeof(wfc) * (size_t)n * (size_t)nseq;
+    return sizeof(wfc) * (size_t)n * (1 + 2 * (size_t)nseq);
 }
 
 #define WF_DISPATCH_N(n, KERNEL, grd, blk, lds, s, ...)                                                  \
@@ -585,6 +652,15 @@ static size_t wf_fft_lds(int n, int nseq) {
             case 127: hipLaunchKernelGGL(KERNEL<127>, grd, blk, lds, s, __VA_ARGS__); break;            \
             case 255: hipLaunchKernelGGL(KERNEL<255>, grd, blk, lds, s, __VA_ARGS__); break;            \
             case 511: hipLaunchKernelGGL(KERNEL<511>, grd, blk, lds, s, __VA_ARGS__); break;            \
+            default: hipLaunchKernelGGL(KERNEL<0>, grd, blk, lds, s, __VA_ARGS__); break;               \
+        }                                                                                               \
+    } while (0)
+
+#define WF_DISPATCH_F2(n, KERNEL, grd, blk, lds, s, ...)                                                 \
+    do {                                                                                                \
+        switch (n) {                                                                                    \
+            case 63: hipLaunchKernelGGL(KERNEL<63>, grd, blk, lds, s, __VA_ARGS__); break;              \
+            case 255: hipLaunchKernelGGL(KERNEL<255>, grd, blk, lds, s, __VA_ARGS__); break;            \
             default: hipLaunchKernelGGL(KERNEL<0>, grd, blk, lds, s, __VA_ARGS__); break;               \
         }                                                                                               \
     } while (0)
@@ -610,17 +686,11 @@ hipError_t wf_launch_iter_reset(WfIterCtl *ctl, hipStream_t s) {
     return hipGetLastError();
 }
 
-hipError_t wf_launch_iter_check(const WfSpectral &sp, double tol, int m, int max_iter, hipStream_t s) {
-    hipLaunchKernelGGL(k_iter_check, dim3(1), dim3(WF_NT), 0, s, sp.ctl, (const double *)sp.pmax, sp.npmax, tol, m,
-                       max_iter);
-    return hipGetLastError();
-}
-
 hipError_t wf_launch_pressure_rhs_fx(const WfConst &c, const WfSpectral &sp, const WfState &Ys, const WfState &Yn,
                                      const double *p, int mode, hipStream_t s) {
     const int nslabs = (mode == WF_F_TOP) ? 1 : sp.nz - 1;
     dim3 grd((unsigned)(sp.ngroups * nslabs));
-    size_t lds = wf_fft_lds(sp.nx, sp.fx_pairs);
+    size_t lds = wf_fx_lds(sp.nx, sp.fx_pairs);
     switch (mode) {
         case WF_F_ITER: WF_DISPATCH_NM(sp.nx, WF_F_ITER, grd, dim3(WF_NT), lds, s, c, sp, Ys, Yn, p); break;
         case WF_F_FIRST: WF_DISPATCH_NM(sp.nx, WF_F_FIRST, grd, dim3(WF_NT), lds, s, c, sp, Ys, Yn, p); break;
@@ -633,35 +703,39 @@ hipError_t wf_launch_pressure_rhs_fx(const WfConst &c, const WfSpectral &sp, con
 hipError_t wf_launch_fy(const WfSpectral &sp, double2 *buf, long slab_stride, int slab0, int nslabs, int sign,
                         int guarded, hipStream_t s) {
     if (nslabs <= 0) return hipSuccess;
-    dim3 grd((sp.hx + WF_FY_COLS - 1) / WF_FY_COLS, nslabs, 1);
-    size_t lds = wf_fft_lds(sp.ny, WF_FY_COLS);
-    WF_DISPATCH_N(sp.ny, k_fy, grd, dim3(WF_NT), lds, s, sp, buf, slab_stride, slab0, sign, guarded);
+    dim3 grd((sp.hx + sp.fy_cols - 1) / sp.fy_cols, nslabs, 1);
+    size_t lds = wf_fy_lds(sp.ny, sp.fy_cols);
+    WF_DISPATCH_F2(sp.ny, k_fy, grd, dim3(WF_NT), lds, s, sp, buf, slab_stride, slab0, sign, guarded);
     return hipGetLastError();
 }
 
-hipError_t wf_launch_pivots(const WfSpectral &sp, double *L, double *RU, hipStream_t s) {
+hipError_t wf_launch_pivots(const WfSpectral &sp, double *UB, hipStream_t s) {
     dim3 grd((sp.ny + 63) / 64, sp.hx, 1);
-    hipLaunchKernelGGL(k_pivots, grd, dim3(64), 0, s, sp, L, RU);
+    hipLaunchKernelGGL(k_pivots, grd, dim3(64), 0, s, sp, UB);
     return hipGetLastError();
 }
 
 hipError_t wf_launch_thomas(const WfSpectral &sp, hipStream_t s) {
     dim3 grd((sp.ny + 63) / 64, sp.hx, 1);
-    if (sp.nz <= 8 * WF_THOMAS_MAX_CHUNKS) {
-        const int chunks = (sp.nz + 7) / 8;
+    const int chunks = (sp.nz + sp.thomas_lc - 1) / sp.thomas_lc;
+    if (chunks > WF_THOMAS_MAX_CHUNKS) return hipErrorInvalidValue;
+    if (sp.thomas_lc == 8)
         hipLaunchKernelGGL(k_thomas<8>, grd, dim3(64 * chunks), 0, s, sp);
-    } else if (sp.nz <= 16 * WF_THOMAS_MAX_CHUNKS) {
-        const int chunks = (sp.nz + 15) / 16;
+    else if (sp.thomas_lc == 16)
         hipLaunchKernelGGL(k_thomas<16>, grd, dim3(64 * chunks), 0, s, sp);
-    } else {
+    else
         return hipErrorInvalidValue;
-    }
+    return hipGetLastError();
+}
+
+hipError_t wf_launch_iter_check(const WfSpectral &sp, double tol, int m, int max_iter, hipStream_t s) {
+    hipLaunchKernelGGL(k_iter_check, dim3(1), dim3(WF_NT), 0, s, sp, tol, m, max_iter);
     return hipGetLastError();
 }
 
 hipError_t wf_launch_inverse_fx(const WfConst &c, const WfSpectral &sp, double *p, hipStream_t s) {
     dim3 grd((unsigned)sp.npmax);
-    size_t lds = wf_fft_lds(sp.nx, sp.fx_pairs);
+    size_t lds = wf_fx_lds(sp.nx, sp.fx_pairs);
     WF_DISPATCH_N(sp.nx, k_inv_fx, grd, dim3(WF_NT), lds, s, c, sp, p);
     return hipGetLastError();
 }

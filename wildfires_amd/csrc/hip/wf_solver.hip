@@ -55,7 +55,7 @@ struct wf_solver {
     double *state[4];
     int i_yn, i_x0, i_x1, i_acc;
     double *derived;
-    double *p, *A, *L, *RU, *pmax;
+    double *p, *A, *UB, *pmax;
     double2 *F, *Ptop;
     double *z, *cd2, *zc, *topo, *kx2, *ky2, *ta, *tc, *beta, *x, *y;
     int *cut, *nzy, *radx, *rady;
@@ -63,6 +63,8 @@ struct wf_solver {
     double *scratch, *Tsrc, *rho;
     int have_Tsrc;
     WfIterCtl *ctl;
+    int *hflag;      /* mapped pinned host memory: [0] done, [1] solves finished */
+    int *hflag_dev;  /* its device address */
     int method; /* 0 Euler, 1 RK2, 2 RK4 */
     long press_solves, steps;
     double last_err;
@@ -224,6 +226,10 @@ static int wf_create_impl(wf_solver *s, const Parameters *P) {
     WF_HIP(hipMalloc((void **)&s->rho, sizeof(double) * s->N));
     WF_HIP(hipMalloc((void **)&s->ctl, sizeof(WfIterCtl)));
     WF_HIP(hipMemset(s->ctl, 0, sizeof(WfIterCtl)));
+    WF_HIP(hipHostMalloc((void **)&s->hflag, 2 * sizeof(int), hipHostMallocMapped));
+    WF_HIP(hipHostGetDevicePointer((void **)&s->hflag_dev, s->hflag, 0));
+    s->hflag[0] = 0;
+    s->hflag[1] = 0;
 
     /* grid tables evaluated on the host exactly as the reference evaluates them */
     const double *z = P->z;
@@ -358,7 +364,7 @@ static int wf_create_impl(wf_solver *s, const Parameters *P) {
     sp.hx = sp.nx / 2 + 1;
     sp.ldj = (sp.ny + 7) & ~7;
     sp.fx_pairs = wf_pairs_for(sp.nx);
-    sp.fy_cols = WF_FY_COLS;
+    sp.fy_cols = wf_f2_n1(sp.ny) ? wf_f2_nseq(sp.ny) : WF_FY_COLS;
     sp.ngroups = ((sp.ny + 1) / 2 + sp.fx_pairs - 1) / sp.fx_pairs;
     sp.npmax = sp.ngroups * sp.nz + 1;
     int radx[WF_MAX_RADICES], rady[WF_MAX_RADICES];
@@ -408,15 +414,16 @@ static int wf_create_impl(wf_solver *s, const Parameters *P) {
     WF_HIP(hipMemset(s->F, 0, sizeof(double2) * nspec));
     WF_HIP(hipMalloc((void **)&s->Ptop, sizeof(double2) * (long)sp.hx * sp.ldj));
     WF_HIP(hipMemset(s->Ptop, 0, sizeof(double2) * (long)sp.hx * sp.ldj));
-    WF_HIP(hipMalloc((void **)&s->L, sizeof(double) * nspec));
-    WF_HIP(hipMalloc((void **)&s->RU, sizeof(double) * nspec));
+    sp.thomas_lc = (sp.nz <= 8 * WF_THOMAS_MAX_CHUNKS) ? 8 : 16;
+    const int nchunks = (sp.nz + sp.thomas_lc - 1) / sp.thomas_lc;
+    WF_HIP(hipMalloc((void **)&s->UB, sizeof(double) * (long)nchunks * sp.hx * sp.ldj));
+    WF_HIP(hipMemset(s->UB, 0, sizeof(double) * (long)nchunks * sp.hx * sp.ldj));
     WF_HIP(hipMalloc((void **)&s->pmax, sizeof(double) * sp.npmax));
     WF_HIP(hipMemset(s->pmax, 0, sizeof(double) * sp.npmax));
     sp.F = s->F;
     sp.Ptop = s->Ptop;
     sp.A = s->A;
-    sp.L = s->L;
-    sp.RU = s->RU;
+    sp.UB = s->UB;
     sp.pmax = s->pmax;
     sp.kx2 = s->kx2;
     sp.ky2 = s->ky2;
@@ -429,7 +436,8 @@ static int wf_create_impl(wf_solver *s, const Parameters *P) {
     sp.rady = s->rady;
     sp.rho = s->rho;
     sp.ctl = s->ctl;
-    WF_HIP(wf_launch_pivots(sp, s->L, s->RU, s->stream));
+    sp.hflag = s->hflag_dev;
+    WF_HIP(wf_launch_pivots(sp, s->UB, s->stream));
     WF_HIP(hipStreamSynchronize(s->stream));
     for (int q = 0; q < 2 * WF_MAX_EVENTS; q++) WF_HIP(hipEventCreate(&s->ev[q]));
     return WF_OK;
@@ -456,13 +464,14 @@ extern "C" int wf_solver_destroy(wf_solver *s) {
     if (s == NULL) return WF_OK;
     if (s->stream) (void)hipStreamSynchronize(s->stream);
     for (int q = 0; q < 4; q++) (void)hipFree(s->state[q]);
-    void *bufs[] = {s->zc, s->derived, s->p, s->A, s->L, s->RU, s->pmax, s->F, s->Ptop, s->z, s->cd2, s->topo, s->kx2, s->ky2, s->ta,
+    void *bufs[] = {s->zc, s->derived, s->p, s->A, s->UB, s->pmax, s->F, s->Ptop, s->z, s->cd2, s->topo, s->kx2, s->ky2, s->ta,
                     s->tc, s->beta, s->x, s->y, s->cut, s->nzy, s->radx, s->rady, s->twx, s->twy, s->scratch,
                     s->Tsrc, s->rho, s->ctl};
     for (size_t q = 0; q < sizeof(bufs) / sizeof(bufs[0]); q++)
         if (bufs[q]) (void)hipFree(bufs[q]);
     for (int q = 0; q < 2 * WF_MAX_EVENTS; q++)
         if (s->ev[q]) (void)hipEventDestroy(s->ev[q]);
+    if (s->hflag) (void)hipHostFree(s->hflag);
     if (s->stream) (void)hipStreamDestroy(s->stream);
     free(s);
     return WF_OK;
@@ -571,10 +580,12 @@ static int wf_integrate(wf_solver *s) {
 }
 
 /* Pressure solve of src/cu/pressure.cu:506-606 for y* = Ys, y_n = Yn.  The
- * solves are queued without host synchronisation; the convergence test runs
- * on the device after each solve and turns the remaining kernels into no-ops.
- * The host looks at the flag every WF_POLL solves to stop queueing. */
-#define WF_POLL 8
+ * solves are queued without host synchronisation; k_iter_check tests
+ * convergence on the device after each solve, turns the remaining kernels
+ * into no-ops and writes the flag and the solve count to mapped host memory.  The
+ * host keeps at most WF_LOOKAHEAD solves queued ahead of the device and stops
+ * queueing once the flag is set. */
+#define WF_LOOKAHEAD 6
 static int wf_pressure_solve(wf_solver *s, double *Ys_buf, double *Yn_buf) {
     const Parameters &P = s->P;
     WfState ys = wf_view(s, Ys_buf), yn = wf_view(s, Yn_buf);
@@ -585,22 +596,26 @@ static int wf_pressure_solve(wf_solver *s, double *Ys_buf, double *Yn_buf) {
         s->last_iters = 0;
         return WF_OK;
     }
+    volatile int *hf = s->hflag;
+    hf[0] = 0;
+    hf[1] = 0;
     if (P.variable_density) WF_LAUNCH(s, WF_K_OTHER, wf_launch_density(s->c, yn.T, s->rho, s->stream));
+    const long sstride = (long)s->sp.hx * s->sp.ldj;
     for (int m = 0; m < max_iter; m++) {
         const int form = (m > 0) ? WF_F_ITER : (P.variable_density ? WF_F_FIRST : WF_F_CONST);
-        const long sstride = (long)s->sp.hx * s->sp.ldj;
         WF_LAUNCH(s, WF_K_PRHS, wf_launch_pressure_rhs_fx(s->c, s->sp, ys, yn, s->p, form, s->stream));
         WF_LAUNCH(s, WF_K_PFY, wf_launch_fy(s->sp, s->F, sstride, 1, s->sp.nz - 1, -1, 1, s->stream));
         WF_LAUNCH(s, WF_K_THOMAS, wf_launch_thomas(s->sp, s->stream));
         WF_LAUNCH(s, WF_K_PFY, wf_launch_fy(s->sp, s->F, sstride, 0, s->sp.nz, +1, 1, s->stream));
         WF_LAUNCH(s, WF_K_PINV, wf_launch_inverse_fx(s->c, s->sp, s->p, s->stream));
         WF_LAUNCH(s, WF_K_OTHER, wf_launch_iter_check(s->sp, P.pressure_solver_tol, m, max_iter, s->stream));
-        if ((m + 1) % WF_POLL == 0 && m + 1 < max_iter) {
-            int done = 0;
-            WF_HIP(hipMemcpyAsync(&done, &s->ctl->done, sizeof(int), hipMemcpyDeviceToHost, s->stream));
-            WF_HIP(hipStreamSynchronize(s->stream));
-            if (done) break;
+        if (m + 1 >= max_iter) break;
+        /* wait until the device is within WF_LOOKAHEAD solves; if the stream
+         * drains without the flag advancing, fall through and keep queueing */
+        while (!hf[0] && (m + 1) - hf[1] >= WF_LOOKAHEAD) {
+            if (hipStreamQuery(s->stream) == hipSuccess) break;
         }
+        if (hf[0]) break;
     }
     WfIterCtl ctl;
     WF_HIP(hipMemcpyAsync(&ctl, s->ctl, sizeof(ctl), hipMemcpyDeviceToHost, s->stream));
