@@ -252,7 +252,7 @@ def _kernel_class(name):
         return None
     base = short.split("<")[0]
     for prefix, cls in KERNEL_CLASSES:
-        if base == prefix or (prefix == "k_stage" and base == "k_stage_columns"):
+        if base == prefix or (prefix == "k_stage" and base in ("k_stage_columns", "k_stage_dup")):
             return cls
     return None
 
@@ -301,9 +301,10 @@ def traffic_json(pmc_dir, cal_dir, copy_bytes):
             if cls is None:
                 continue
             a = acc.setdefault(cls, {"FETCH_SIZE": [0.0, 0], "WRITE_SIZE": [0.0, 0]})
-            # k_stage_columns is a companion launch of k_stage: add, do not count
+            # k_stage_dup and k_stage_columns are companion launches of
+            # k_stage: their bytes add to its launch, they do not count
             a[counter][0] += v * 1024.0 * scales[counter]
-            if not _short_name(name).startswith("k_stage_columns"):
+            if not _short_name(name).startswith(("k_stage_columns", "k_stage_dup")):
                 a[counter][1] += 1
     kernels = {}
     for cls, a in sorted(acc.items()):

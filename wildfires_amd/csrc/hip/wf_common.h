@@ -161,17 +161,18 @@ __device__ __forceinline__ int wf_xcd_tile(int b, int nblocks) {
 #ifndef WF_TILE_ORDER
 #define WF_TILE_ORDER 0
 #endif
-__device__ __forceinline__ void wf_tile_decode(int ncol, int nk, int *col, int *k) {
+/* tile of block b of a launch of nb tile blocks */
+__device__ __forceinline__ void wf_tile_decode_at(int b, int nb, int ncol, int nk, int *col, int *k) {
 #if WF_TILE_ORDER == 0
-    const int t = blockIdx.x;
+    const int t = b;
     *col = t % ncol;
     *k = t / ncol;
 #elif WF_TILE_ORDER == 2
-    const int t = wf_xcd_tile(blockIdx.x, gridDim.x);
+    const int t = wf_xcd_tile(b, nb);
     *col = t % ncol;
     *k = t / ncol;
 #elif WF_TILE_ORDER == 3
-    const int t = wf_xcd_tile(blockIdx.x, gridDim.x);
+    const int t = wf_xcd_tile(b, nb);
     const int nfull = nk >> 3;
     if (t < nfull * 8 * ncol) {
         const int g = t / (8 * ncol), r = t - g * 8 * ncol;
@@ -183,10 +184,13 @@ __device__ __forceinline__ void wf_tile_decode(int ncol, int nk, int *col, int *
         *k = nfull * 8 + r % gs;
     }
 #else
-    const int t = wf_xcd_tile(blockIdx.x, gridDim.x);
+    const int t = wf_xcd_tile(b, nb);
     *k = t % nk;
     *col = t / nk;
 #endif
+}
+__device__ __forceinline__ void wf_tile_decode(int ncol, int nk, int *col, int *k) {
+    wf_tile_decode_at(blockIdx.x, gridDim.x, ncol, nk, col, k);
 }
 
 __host__ __device__ __forceinline__ int wf_tile_count(const WfConst &c) {

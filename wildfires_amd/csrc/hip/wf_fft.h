@@ -319,10 +319,19 @@ __device__ __forceinline__ void wf_fft_ip(wfc *buf, const wfc *tw) {
 /* Factor pair of the lengths with a two-factor plan (0 if none). */
 __host__ __device__ constexpr int wf_f2_n1(int n) { return n == 255 ? 15 : n == 63 ? 7 : 0; }
 __host__ __device__ constexpr int wf_f2_n2(int n) { return n == 255 ? 17 : n == 63 ? 9 : 0; }
-/* sequences per 256-thread workgroup */
+/* sequences per workgroup: the most that max(N1, N2) threads each fit in
+ * 256 threads, or WF_FY_NSEQ255 for length 255 (fewer sequences, less LDS
+ * and a smaller workgroup, so more workgroups per CU) */
+#ifndef WF_FY_NSEQ255
+#define WF_FY_NSEQ255 12
+#endif
+__host__ __device__ constexpr int wf_f2_nmax(int n) { return wf_f2_n1(n) > wf_f2_n2(n) ? wf_f2_n1(n) : wf_f2_n2(n); }
 __host__ __device__ constexpr int wf_f2_nseq(int n) {
-    return wf_f2_n1(n) ? 256 / (wf_f2_n1(n) > wf_f2_n2(n) ? wf_f2_n1(n) : wf_f2_n2(n)) : 0;
+    return !wf_f2_n1(n) ? 0 : (n == 255 && WF_FY_NSEQ255 > 0) ? WF_FY_NSEQ255 : 256 / wf_f2_nmax(n);
 }
+/* threads of a two-factor workgroup: whole waves covering both phases */
+__host__ __device__ constexpr int wf_f2_threads(int n) { return (wf_f2_nmax(n) * wf_f2_nseq(n) + 63) / 64 * 64; }
+static_assert(wf_f2_threads(255) <= 256 && wf_f2_threads(63) <= 256, "two-factor workgroup above 256 threads");
 
 /* Odd R-point DFT of x (registers) with W = exp(SIGN 2 pi i / R) taken from
  * the forward N-point table tw (W_R^m = tw[m TS] or its conjugate), outputs
@@ -372,13 +381,18 @@ template <int N1, int N2, int SIGN, class Load>
 __device__ __forceinline__ void wf_f2_phase_a(wfc *buf, const double2 *tw, int s, int n1, Load &&load) {
 #pragma clang fp contract(fast)
     constexpr int N = N1 * N2;
+    /* this row's twiddles W_N^{n1 k2} (n1 k2 < N) are loaded together with
+     * its inputs, not one by one where they are used */
+    double2 wt[N2];
+#pragma unroll
+    for (int k2 = 1; k2 < N2; k2++) wt[k2] = tw[n1 * k2];
     wfc x[N2];
 #pragma unroll
     for (int n2 = 0; n2 < N2; n2++) x[n2] = load(s, n1 + N1 * n2);
     wfc *row = buf + s * N + n1;
     wf_dft_reg<N2, N1, SIGN>(x, tw, [&](int k2, wfc y) {
-        if (k2 != 0) { /* twiddle W_N^{n1 k2}; n1 k2 < N */
-            const double2 W = tw[n1 * k2];
+        if (k2 != 0) {
+            const double2 W = wt[k2];
             const double wi = (SIGN < 0) ? W.y : -W.y;
             wfc t;
             t.re = y.re * W.x - y.im * wi;

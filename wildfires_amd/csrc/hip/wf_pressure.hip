@@ -185,8 +185,20 @@ __global__ void k_iter_reset(WfIterCtl *ctl) {
 __global__ __launch_bounds__(WF_NT) void k_iter_check(WfSpectral sp, double tol, int m, int max_iter) {
     WfIterCtl *ctl = sp.ctl;
     if (ctl->done) return;
+    /* 16 independent loads per thread and pass, so one pass costs one memory
+     * round trip (a plain strided loop waits for each load in turn) */
+    constexpr int U = 16;
     double v = 0.0;
-    for (int t = threadIdx.x; t < sp.npmax; t += blockDim.x) v = (sp.pmax[t] > v) ? sp.pmax[t] : v;
+    for (int t0 = 0; t0 < sp.npmax; t0 += U * WF_NT) {
+        double x[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int t = t0 + u * WF_NT + (int)threadIdx.x;
+            x[u] = (t < sp.npmax) ? sp.pmax[t] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < U; u++) v = (x[u] > v) ? x[u] : v;
+    }
     __shared__ double err;
     wf_block_max_store(v, &err);
     __syncthreads();
@@ -246,7 +258,27 @@ __global__ __launch_bounds__(WF_NT) void k_rhs_fx(WfConst c, WfSpectral sp, WfSt
     const int j0 = 2 * g * pairs;
     for (int t = threadIdx.x; t < nx; t += blockDim.x) tw[t] = *(const wfc *)&sp.twx[t];
     if (NX == 0 && threadIdx.x < WF_MAX_RADICES) rad[threadIdx.x] = sp.radx[threadIdx.x];
-    if constexpr (NX > 0) {
+    if constexpr (NX > 0 && MODE != WF_F_FIRST) {
+        /* one row pair per batch: both cells' loads unconditional (indices
+         * clamped to a valid node, padding results dropped), so a batch is
+         * one basic block with all 28 loads in flight; the scheduling
+         * barrier keeps the batches apart, which bounds the registers */
+        constexpr int PER = (PAIRS * NX + WF_NT - 1) / WF_NT;
+#pragma unroll
+        for (int pi = 0; pi < PER; pi++) {
+            const int t = threadIdx.x + pi * WF_NT;
+            const int tc = (t < PAIRS * NX) ? t : PAIRS * NX - 1;
+            const int s = tc / NX, i = tc - s * NX;
+            const int ja = j0 + 2 * s, jb = ja + 1;
+            const double fa = wf_fx_input<MODE>(c, sp, Ys, Yn, p, i, (ja < ny) ? ja : ny - 1, k);
+            const double fb = wf_fx_input<MODE>(c, sp, Ys, Yn, p, i, (jb < ny) ? jb : ny - 1, k);
+            if (t < PAIRS * NX) {
+                bufa[t].re = (ja < ny) ? fa : 0.0;
+                bufa[t].im = (jb < ny) ? fb : 0.0;
+            }
+            __builtin_amdgcn_sched_barrier(0);
+        }
+    } else if constexpr (NX > 0) {
         constexpr int PER = (PAIRS * NX + WF_NT - 1) / WF_NT;
 #pragma unroll
         for (int pi = 0; pi < PER; pi++) {
@@ -291,13 +323,14 @@ __global__ __launch_bounds__(WF_NT) void k_rhs_fx(WfConst c, WfSpectral sp, WfSt
 /* P2/P4: y-transform of sp.fy_cols kx columns (each contiguous over ky) of
  * slab slab0 + blockIdx.y; sign -1 forward, +1 inverse. */
 template <int NY, int SIGN>
-__device__ __forceinline__ void wf_fy_f2(const WfSpectral &sp, double2 *base, wfc *buf, int r0) {
+__device__ __forceinline__ void wf_fy_f2(const WfSpectral &sp, double2 *base, wfc *buf, int r0,
+                                         const double2 *__restrict__ twy) {
     constexpr int N1 = wf_f2_n1(NY), N2 = wf_f2_n2(NY), NSEQ = wf_f2_nseq(NY);
     const int hx = sp.hx, ldj = sp.ldj;
     const int t = threadIdx.x;
     if (t < N1 * NSEQ) {
         const int s = t / N1, n1 = t - s * N1;
-        wf_f2_phase_a<N1, N2, SIGN>(buf, sp.twy, s, n1, [&](int ss, int n) {
+        wf_f2_phase_a<N1, N2, SIGN>(buf, twy, s, n1, [&](int ss, int n) {
             wfc v = {0.0, 0.0};
             if (r0 + ss < hx) {
                 const double2 x = base[(long)(r0 + ss) * ldj + n];
@@ -312,14 +345,17 @@ __device__ __forceinline__ void wf_fy_f2(const WfSpectral &sp, double2 *base, wf
         const int s = t / N2, k2 = t - s * N2;
         if (r0 + s < hx) {
             double2 *col = base + (long)(r0 + s) * ldj + k2;
-            wf_f2_phase_b<N1, N2, SIGN>(buf, sp.twy, s, k2, [&](int k1, wfc X) { col[N2 * k1] = make_double2(X.re, X.im); });
+            wf_f2_phase_b<N1, N2, SIGN>(buf, twy, s, k2, [&](int k1, wfc X) { col[N2 * k1] = make_double2(X.re, X.im); });
         }
     }
 }
 
+/* twy: the y twiddle table again, as a read-only argument of its own, so
+ * that the compiler may keep its loads apart from the column stores (scalar
+ * loads for the DFT constants, vector loads issued early for the rest). */
 template <int NY>
 __global__ __launch_bounds__(WF_NT) void k_fy(WfSpectral sp, double2 *buf, long slab_stride, int slab0, int sign,
-                                              int guarded) {
+                                              int guarded, const double2 *__restrict__ twy) {
     if (guarded && sp.ctl->done) return;
     extern __shared__ wfc smem[];
     double2 *base = buf + (long)(slab0 + blockIdx.y) * slab_stride;
@@ -327,9 +363,9 @@ __global__ __launch_bounds__(WF_NT) void k_fy(WfSpectral sp, double2 *buf, long 
     const int r0 = blockIdx.x * cols;
     if constexpr (NY > 0) {
         if (sign < 0)
-            wf_fy_f2<NY, -1>(sp, base, smem, r0);
+            wf_fy_f2<NY, -1>(sp, base, smem, r0, twy);
         else
-            wf_fy_f2<NY, 1>(sp, base, smem, r0);
+            wf_fy_f2<NY, 1>(sp, base, smem, r0, twy);
     } else {
         __shared__ int rad[WF_MAX_RADICES];
         const int ny = sp.ny, hx = sp.hx, ldj = sp.ldj;
@@ -526,9 +562,10 @@ __global__ __launch_bounds__(64 * WF_THOMAS_MAX_CHUNKS) void k_thomas(WfSpectral
 
 /* new value pv of node (i, j, k) and of its periodic duplicates, with the
  * running max |p_new - p_old| */
-__device__ __forceinline__ void wf_put_p(const WfConst &c, double *p, int i, int j, int k, double pv, double &dmax) {
+__device__ __forceinline__ void wf_put_p_old(const WfConst &c, double *p, int i, int j, int k, double pv, double pold,
+                                             double &dmax) {
     const long q = wf_q(c, i, j, k);
-    const double d = fabs(pv - p[q]);
+    const double d = fabs(pv - pold);
     dmax = (d > dmax) ? d : dmax;
     p[q] = pv;
     if (i == 0) { /* duplicate column i = Nx-1 */
@@ -547,6 +584,134 @@ __device__ __forceinline__ void wf_put_p(const WfConst &c, double *p, int i, int
             const double dc = fabs(pv - p[qc]);
             dmax = (dc > dmax) ? dc : dmax;
             p[qc] = pv;
+        }
+    }
+}
+__device__ __forceinline__ void wf_put_p(const WfConst &c, double *p, int i, int j, int k, double pv, double &dmax) {
+    wf_put_p_old(c, p, i, j, k, pv, p[wf_q(c, i, j, k)], dmax);
+}
+
+/* Compile-time plan of P5: every global load (half spectra, the old p of the
+ * group's unique nodes and of their periodic duplicates) is issued before the
+ * transform, unconditionally with clamped indices, so a workgroup waits for
+ * one memory round trip; the stores of the duplicates use no branches inside
+ * the main loop (column i = Nx-1 from threads 0 .. 2*PAIRS-1, row j = Ny-1 by
+ * the group that holds j = 0).  The set of (node, new value) pairs, and so
+ * the norm, is the one of wf_put_p. */
+template <int NX>
+__device__ __forceinline__ void wf_inv_fx_planned(const WfConst &c, const WfSpectral &sp, double *p, wfc *smem,
+                                                  int g, int k, double &dmax) {
+    constexpr int PAIRS = wf_pairs_for(NX);
+    constexpr int PER = (PAIRS * NX + WF_NT - 1) / WF_NT;
+    constexpr int PERY = (NX + 1 + WF_NT - 1) / WF_NT; /* row j = Ny-1: i = 0 .. Nx-1 */
+    const int ny = sp.ny, hx = sp.hx, ldj = sp.ldj;
+    const int j0 = 2 * g * PAIRS;
+    const double2 *src = sp.F + (long)k * hx * ldj;
+    const double inv_scale = 1.0 / ((double)NX * (double)ny);
+    const int tid = threadIdx.x;
+    wfc *tw = smem, *buf = smem + NX;
+    /* loads: half spectra (thread t -> pair t % PAIRS, bin t / PAIRS, so
+     * consecutive threads read consecutive ky of one kx column) */
+    double2 xa[PER], xb[PER];
+#pragma unroll
+    for (int pi = 0; pi < PER; pi++) {
+        const int t = tid + pi * WF_NT;
+        const int tc = (t < PAIRS * NX) ? t : PAIRS * NX - 1;
+        const int s = tc % PAIRS, r = tc / PAIRS;
+        const int ja = j0 + 2 * s, jb = ja + 1;
+        const int rr = (r >= hx) ? NX - r : r;
+        xa[pi] = src[(long)rr * ldj + ((ja < ny) ? ja : ny - 1)];
+        xb[pi] = src[(long)rr * ldj + ((jb < ny) ? jb : ny - 1)];
+    }
+    /* old p of the unique nodes (thread t -> pair t / NX, i = t % NX) */
+    double pa[PER], pb[PER];
+#pragma unroll
+    for (int pi = 0; pi < PER; pi++) {
+        const int t = tid + pi * WF_NT;
+        const int tc = (t < PAIRS * NX) ? t : PAIRS * NX - 1;
+        const int s = tc / NX, i = tc - s * NX;
+        const int ja = j0 + 2 * s, jb = ja + 1;
+        pa[pi] = p[wf_q(c, i, (ja < ny) ? ja : ny - 1, k)];
+        pb[pi] = p[wf_q(c, i, (jb < ny) ? jb : ny - 1, k)];
+    }
+    /* old p of the duplicates: column i = Nx-1 of the group's rows, and
+     * row j = Ny-1 (i = 0 .. Nx-1) when the group holds j = 0 */
+    const int jr = j0 + ((tid < 2 * PAIRS) ? tid : 0);
+    const double pdx = p[wf_q(c, c.Nx - 1, (jr < ny) ? jr : ny - 1, k)];
+    double pdy[PERY];
+    if (g == 0) {
+#pragma unroll
+        for (int pi = 0; pi < PERY; pi++) {
+            const int i = tid + pi * WF_NT;
+            pdy[pi] = p[wf_q(c, (i <= NX) ? i : NX, c.Ny - 1, k)];
+        }
+    }
+    for (int t = tid; t < NX; t += WF_NT) {
+        const double2 w = sp.twx[t];
+        tw[t].re = w.x;
+        tw[t].im = -w.y;
+    }
+#pragma unroll
+    for (int pi = 0; pi < PER; pi++) {
+        const int t = tid + pi * WF_NT;
+        const int s = t % PAIRS, r = t / PAIRS;
+        const int ja = j0 + 2 * s, jb = ja + 1;
+        if (t < PAIRS * NX) {
+            wfc A = {0.0, 0.0}, B = {0.0, 0.0};
+            if (ja < ny) {
+                A.re = xa[pi].x;
+                A.im = xa[pi].y;
+                if (jb < ny) {
+                    B.re = xb[pi].x;
+                    B.im = xb[pi].y;
+                }
+            }
+            buf[s * NX + r] = wf_pack_pair(A, B, r >= hx);
+        }
+    }
+    __syncthreads();
+    wf_fft_ip<NX, PAIRS, WF_NT>(buf, tw);
+    /* unique nodes */
+#pragma unroll
+    for (int pi = 0; pi < PER; pi++) {
+        const int t = tid + pi * WF_NT;
+        const int s = t / NX, i = t - s * NX;
+        const int ja = j0 + 2 * s, jb = ja + 1;
+        if (t < PAIRS * NX) {
+            const wfc v = buf[s * NX + i];
+            if (ja < ny) {
+                const double pv = v.re * inv_scale;
+                const double d = fabs(pv - pa[pi]);
+                dmax = (d > dmax) ? d : dmax;
+                p[wf_q(c, i, ja, k)] = pv;
+            }
+            if (jb < ny) {
+                const double pv = v.im * inv_scale;
+                const double d = fabs(pv - pb[pi]);
+                dmax = (d > dmax) ? d : dmax;
+                p[wf_q(c, i, jb, k)] = pv;
+            }
+        }
+    }
+    /* duplicate column i = Nx-1 of rows j0 .. j0 + 2 PAIRS - 1 (value of i = 0) */
+    if (tid < 2 * PAIRS && jr < ny) {
+        const wfc v = buf[(tid >> 1) * NX];
+        const double pv = ((tid & 1) ? v.im : v.re) * inv_scale;
+        const double d = fabs(pv - pdx);
+        dmax = (d > dmax) ? d : dmax;
+        p[wf_q(c, c.Nx - 1, jr, k)] = pv;
+    }
+    /* duplicate row j = Ny-1 (value of j = 0; its corner i = Nx-1 that of i = 0) */
+    if (g == 0) {
+#pragma unroll
+        for (int pi = 0; pi < PERY; pi++) {
+            const int i = tid + pi * WF_NT;
+            if (i <= NX) {
+                const double pv = buf[(i < NX) ? i : 0].re * inv_scale;
+                const double d = fabs(pv - pdy[pi]);
+                dmax = (d > dmax) ? d : dmax;
+                p[wf_q(c, i, c.Ny - 1, k)] = pv;
+            }
         }
     }
 }
@@ -581,11 +746,12 @@ __global__ __launch_bounds__(WF_NT) void k_inv_fx(WfConst c, WfSpectral sp, doub
     }
     int g, k;
     wf_tile_decode(sp.ngroups, sp.nz, &g, &k);
-    const int j0 = 2 * g * pairs;
-    const double2 *src = sp.F + (long)k * hx * ldj;
-    const double inv_scale = 1.0 / ((double)nx * (double)ny);
-    {
-        constexpr int PAIRS = (NX > 0) ? wf_pairs_for(NX) : 0;
+    if constexpr (NX > 0) {
+        wf_inv_fx_planned<NX>(c, sp, p, smem, g, k, dmax);
+    } else {
+        const int j0 = 2 * g * pairs;
+        const double2 *src = sp.F + (long)k * hx * ldj;
+        const double inv_scale = 1.0 / ((double)nx * (double)ny);
         const int Nx = c.Nx;
         wfc *tw = smem, *bufa = smem + nx, *bufb = bufa + pairs * nx;
         for (int t = threadIdx.x; t < nx; t += blockDim.x) {
@@ -593,9 +759,7 @@ __global__ __launch_bounds__(WF_NT) void k_inv_fx(WfConst c, WfSpectral sp, doub
             tw[t].re = w.x;
             tw[t].im = -w.y;
         }
-        if (NX == 0 && threadIdx.x < WF_MAX_RADICES) rad[threadIdx.x] = sp.radx[threadIdx.x];
-        /* thread t -> pair s = t % pairs, bin r = t / pairs: consecutive threads
-         * read consecutive ky of one kx column */
+        if (threadIdx.x < WF_MAX_RADICES) rad[threadIdx.x] = sp.radx[threadIdx.x];
         for (int t = threadIdx.x; t < pairs * nx; t += blockDim.x) {
             const int s = t % pairs, r = t / pairs;
             const int ja = j0 + 2 * s, jb = ja + 1;
@@ -615,7 +779,7 @@ __global__ __launch_bounds__(WF_NT) void k_inv_fx(WfConst c, WfSpectral sp, doub
             bufa[s * nx + r] = wf_pack_pair(A, B, conj);
         }
         __syncthreads();
-        wfc *res = wf_transform<NX, PAIRS>(bufa, bufb, nx, pairs, rad, tw);
+        wfc *res = wf_lds_fft(bufa, bufb, nx, pairs, rad, tw);
         for (int t = threadIdx.x; t < pairs * (Nx - 1); t += blockDim.x) {
             const int s = t / (Nx - 1), i = t - s * (Nx - 1);
             const int ja = j0 + 2 * s, jb = ja + 1;
@@ -705,7 +869,9 @@ hipError_t wf_launch_fy(const WfSpectral &sp, double2 *buf, long slab_stride, in
     if (nslabs <= 0) return hipSuccess;
     dim3 grd((sp.hx + sp.fy_cols - 1) / sp.fy_cols, nslabs, 1);
     size_t lds = wf_fy_lds(sp.ny, sp.fy_cols);
-    WF_DISPATCH_F2(sp.ny, k_fy, grd, dim3(WF_NT), lds, s, sp, buf, slab_stride, slab0, sign, guarded);
+    const int nt = wf_f2_n1(sp.ny) ? wf_f2_threads(sp.ny) : WF_NT;
+    WF_DISPATCH_F2(sp.ny, k_fy, grd, dim3(nt), lds, s, sp, buf, slab_stride, slab0, sign, guarded,
+                   (const double2 *)sp.twy);
     return hipGetLastError();
 }
 

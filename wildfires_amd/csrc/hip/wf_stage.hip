@@ -9,8 +9,8 @@
  * (src/cu/pde.cu:139-787 plus src/cu/turbulence.cu:255-454) and writes the RK
  * combination (src/cu/utils.cu:14-21, src/cu/solver.cu:14-39) instead of
  * storing the stage derivative; k_stage_dup does the same for the duplicate
- * periodic planes and k_stage_columns completes the cut and top nodes of every
- * column (src/cu/pde.cu:14-99).
+ * periodic planes, and k_stage_columns completes the cut and top nodes of
+ * every column (src/cu/pde.cu:14-99).
  *
  * Arithmetic: the reference's difference formulas with their constant
  * divisors (2 dx, dx^2, the non-uniform z weights, c_p, Pr, ...) taken as
@@ -86,6 +86,12 @@ __global__ __launch_bounds__(256) void k_derived(WfConst c, WfState X, WfDerived
     const double w1 = wf_ld(X.w, kp1), w2 = wf_ld(X.w, kp2), wm1 = wf_ld(X.w, km1), wm2 = wf_ld(X.w, km2);
     const double T1 = wf_ld(X.T, kp1), T2z = wf_ld(X.T, kp2), T3z = wf_ld(X.T, kp3);
     const double Tm1 = wf_ld(X.T, km1), Tm2 = wf_ld(X.T, km2), Tm3 = wf_ld(X.T, km3);
+    /* the fuel column and Y with the other loads: Y keeps Nz_Y_max >= nzy
+     * planes, so the load is unconditional (the k = 0 entry above them, T
+     * when there is no Y) and selected afterwards */
+    const int col = i + Nx * j;
+    const int nzy = c.nzy[col];
+    const double Yld = wf_ld((c.Nzm > 0) ? X.Y : X.T, (k < c.Nzm) ? q : (unsigned)col);
     double uz, vz, wz, Tz, Tzz, T_kp1 = 0.0;
     if (k == 0) { /* one-sided (src/cu/pde.cu:363-365, 497-500) */
         const double *f = zk + WF_Z_D1F, *g = zk + WF_Z_D2F;
@@ -126,11 +132,9 @@ __global__ __launch_bounds__(256) void k_derived(WfConst c, WfState X, WfDerived
                               (wy + vz) * (wy + vz));
     const double mu_sgs = rho * c.cd2[k] * mod_S;
     /* T_RHS_tmp of src/cu/pde.cu:706 (needed for div_U) */
-    const int col = i + Nx * j;
-    const int nzy = c.nzy[col];
     double mu = c.mu, kT = c.kT, dk = 0.0;
     const double K_T = (T > c.T_pc) ? c.A * exp(-c.T_act * invT) : 0.0;
-    const double Ynode = (k < nzy) ? wf_ld(X.Y, q) : 0.0;
+    const double Ynode = (k < nzy) ? Yld : 0.0;
     const double dz_k = (k < Nz - 1) ? c.z[k + 1] - c.z[k] : c.z[k] - c.z[k - 1];
     double Y;
     const double q_src = wf_heat_q(c, k, nzy, T, T_kp1, Ynode, K_T, dz_k, &Y);
@@ -166,6 +170,18 @@ __global__ __launch_bounds__(256) void k_derived(WfConst c, WfState X, WfDerived
 #define WF_DW (WF_TILE_X + 2 * WF_DH)
 #define WF_DHH (WF_TILE_Y + 2 * WF_DH)
 #define WF_NT_STAGE (WF_TILE_X * WF_TILE_Y)
+
+/* Minimum waves per SIMD that k_stage's register allocation must allow
+ * (0: the compiler's choice, which is 2 at 233 VGPRs; 3 caps it at 168 and
+ * spills the rest). */
+#ifndef WF_STAGE_WAVES
+#define WF_STAGE_WAVES 0
+#endif
+#if WF_STAGE_WAVES > 0
+#define WF_STAGE_ATTR __attribute__((amdgpu_waves_per_eu(WF_STAGE_WAVES)))
+#else
+#define WF_STAGE_ATTR
+#endif
 
 struct WfStageTiles {
     double st[4][WF_SHH * WF_SW]; /* u, v, w, T */
@@ -463,6 +479,68 @@ __device__ __forceinline__ void wf_rk_write(int mode, double a, unsigned q, cons
     }
 }
 
+/* Operands of the RK combination of node q (y_n, and the running
+ * combination in modes MID and LAST), loaded before the stage stores anything:
+ * `out` never aliases y_n or the accumulator, so issuing them early lets their
+ * latency hide under the RHS evaluation instead of trailing it, one field at
+ * a time.  Y keeps Nz_Y_max planes: above them the Y operands are read from
+ * the node's k = 0 entry (or from T when there is no Y) and not used. */
+struct WfRkIn {
+    double yn[5], ac[5];
+};
+__device__ __forceinline__ void wf_rk_load(const WfConst &c, int mode, unsigned q, unsigned col, int k,
+                                           const WfState &Yn, const WfState &acc, WfRkIn &in) {
+    const unsigned qy = (k < c.Nzm) ? q : col;
+    if (mode != WF_MODE_PHI) {
+        in.yn[0] = wf_ld(Yn.u, q);
+        in.yn[1] = wf_ld(Yn.v, q);
+        in.yn[2] = wf_ld(Yn.w, q);
+        in.yn[3] = wf_ld(Yn.T, q);
+        in.yn[4] = wf_ld((c.Nzm > 0) ? Yn.Y : Yn.T, qy);
+    }
+    if (mode == WF_MODE_MID || mode == WF_MODE_LAST) {
+        in.ac[0] = wf_ld(acc.u, q);
+        in.ac[1] = wf_ld(acc.v, q);
+        in.ac[2] = wf_ld(acc.w, q);
+        in.ac[3] = wf_ld(acc.T, q);
+        in.ac[4] = wf_ld((c.Nzm > 0) ? acc.Y : acc.T, qy);
+    }
+}
+
+/* The RK combination of wf_rk_write from operands loaded by wf_rk_load. */
+__device__ __forceinline__ void wf_rk_store(int mode, double a, unsigned q, const double r[5], int nf,
+                                            const WfRkIn &in, const WfState &acc, const WfState &out) {
+    double *o[5] = {out.u, out.v, out.w, out.T, out.Y};
+    double *ac[5] = {acc.u, acc.v, acc.w, acc.T, acc.Y};
+#pragma unroll
+    for (int f = 0; f < 5; f++) {
+        if (f >= nf) break;
+        const double kf = r[f];
+        double res;
+        switch (mode) {
+            case WF_MODE_PHI:
+                res = kf;
+                break;
+            case WF_MODE_FIRST:
+                ac[f][q] = kf;
+                res = in.yn[f] + a * kf;
+                break;
+            case WF_MODE_MID:
+                ac[f][q] = in.ac[f] + 2 * kf;
+                res = in.yn[f] + a * kf;
+                break;
+            case WF_MODE_LAST:
+                res = in.yn[f] + a * (in.ac[f] + kf);
+                break;
+            default:
+                res = in.yn[f] + a * kf;
+                break;
+        }
+        o[f][q] = res;
+    }
+}
+
+
 /* Raw RHS of the nodes cut+1, cut+2, Nz-2 and Nz-3, kept per column for the
  * cut and top nodes (k_stage_columns). */
 __device__ __forceinline__ void wf_keep_raw(const WfConst &c, const WfDerived &D, int col, int k, int cut,
@@ -480,19 +558,65 @@ __device__ __forceinline__ void wf_keep_raw(const WfConst &c, const WfDerived &D
     }
 }
 
+/* A node on the duplicate planes i = Nx-1 and j = Ny-1 (t indexes the
+ * Nx + Ny - 1 columns of those planes), neighbours from global memory. */
+__device__ __forceinline__ void wf_stage_dup_node(const WfConst &c, const WfState &X, const WfDerived &D,
+                                                  const WfState &Yn, const WfState &acc, const WfState &out,
+                                                  int mode, double a, int t, int k) {
+    const int i = (t < c.Ny) ? c.Nx - 1 : t - c.Ny;
+    const int j = (t < c.Ny) ? t : c.Ny - 1;
+    const int col = i + c.Nx * j;
+    const int cut = c.cut[col], nzy = c.nzy[col];
+    if (k == cut) return;
+    const long q = wf_q(c, i, j, k);
+    WfRkIn rk;
+    wf_rk_load(c, mode, (unsigned)q, (unsigned)col, k, Yn, acc, rk);
+    double r[5];
+    if (k < cut) {
+        r[0] = c.u_dead;
+        r[1] = c.v_dead;
+        r[2] = c.w_dead;
+        r[3] = c.T_dead;
+        r[4] = (k < nzy) ? c.Y_dead : 0.0;
+    } else {
+        const WfGlobalAcc acc_g{c, X, D, i, j, k};
+        wf_rhs_node(c, acc_g, k, nzy, r);
+        wf_keep_raw(c, D, col, k, cut, r);
+    }
+    wf_rk_store(mode, a, (unsigned)q, r, (k < c.Nzm) ? 5 : 4, rk, acc, out);
+}
+
 /* One stage on the nodes below the top plane: dead nodes (k < cut) take the
  * dead-node values, fluid nodes cut < k <= Nz-2 their RHS; the raw RHS of the
  * nodes cut+1, cut+2, Nz-2 and Nz-3 is also kept per column for the cut and
- * top nodes, which k_stage_columns completes (src/cu/pde.cu:14-99). */
-__global__ __launch_bounds__(WF_NT_STAGE) void k_stage(WfConst c, WfState X, WfDerived D, WfState Yn, WfState acc,
-                                                       WfState out, int mode, double a) {
+ * top nodes, which k_stage_columns completes (src/cu/pde.cu:14-99).
+ * With WF_STAGE_MERGE_DUP the first ndup blocks evaluate the duplicate-plane
+ * nodes one per thread (their tiles cannot hold them, see below), so their
+ * longer-latency waves overlap the tile blocks; that path needs more registers
+ * than the tiles, so by default k_stage_dup evaluates them in a launch of its
+ * own. */
+#ifndef WF_STAGE_MERGE_DUP
+#define WF_STAGE_MERGE_DUP 0
+#endif
+__global__ __launch_bounds__(WF_NT_STAGE) WF_STAGE_ATTR void k_stage(WfConst c, WfState X, WfDerived D,
+                                                                     WfState Yn, WfState acc, WfState out,
+                                                                     int mode, double a, int ndup) {
+    const int tx = threadIdx.x, ty = threadIdx.y, tid = tx + WF_TILE_X * ty;
+#if WF_STAGE_MERGE_DUP
+    if ((int)blockIdx.x < ndup) {
+        const int nd = c.Nx + c.Ny - 1;
+        const int d = (int)blockIdx.x * WF_NT_STAGE + tid;
+        if (d < nd * (c.Nz - 1)) wf_stage_dup_node(c, X, D, Yn, acc, out, mode, a, d % nd, d / nd);
+        return;
+    }
+#endif
     __shared__ WfStageTiles L;
     const int nty = (c.Ny + WF_TILE_Y - 1) / WF_TILE_Y;
     const int ntx = (c.Nx + WF_TILE_X - 1) / WF_TILE_X;
     int colt, k;
-    wf_tile_decode(ntx * nty, c.Nz - 1, &colt, &k); /* the top plane is completed per column */
+    /* the top plane is completed per column */
+    wf_tile_decode_at((int)blockIdx.x - ndup, (int)gridDim.x - ndup, ntx * nty, c.Nz - 1, &colt, &k);
     const int j0 = (colt % nty) * WF_TILE_Y, i0 = (colt / nty) * WF_TILE_X;
-    const int tx = threadIdx.x, ty = threadIdx.y, tid = tx + WF_TILE_X * ty;
     const int i = i0 + tx, j = j0 + ty;
     const bool inside = (i < c.Nx) && (j < c.Ny);
     const unsigned P = (unsigned)c.plane;
@@ -557,6 +681,8 @@ __global__ __launch_bounds__(WF_NT_STAGE) void k_stage(WfConst c, WfState X, WfD
         V.topo = wf_ld(c.topo, (unsigned)col);
     }
     const int cut = c.cut[col], nzy = c.nzy[col];
+    WfRkIn rk;
+    wf_rk_load(c, mode, q, (unsigned)col, k, Yn, acc, rk);
     __syncthreads();
     /* the duplicate planes i = Nx-1, j = Ny-1 read their x- (y-) neighbours
      * in their own row (column), which the tiles hold as row (column) 0:
@@ -574,35 +700,16 @@ __global__ __launch_bounds__(WF_NT_STAGE) void k_stage(WfConst c, WfState X, WfD
         wf_rhs_node(c, acc_t, k, nzy, r);
         wf_keep_raw(c, D, col, k, cut, r);
     }
-    wf_rk_write(mode, a, q, r, (k < c.Nzm) ? 5 : 4, Yn, acc, out);
+    wf_rk_store(mode, a, q, r, (k < c.Nzm) ? 5 : 4, rk, acc, out);
 }
 
-/* The nodes of k_stage on the duplicate planes i = Nx-1 and j = Ny-1, one
- * thread per node, neighbours from global memory. */
+/* The duplicate-plane nodes of k_stage in a launch of their own: one thread
+ * per node, (Nx + Ny - 1) columns x (Nz - 1) planes. */
 __global__ __launch_bounds__(256) void k_stage_dup(WfConst c, WfState X, WfDerived D, WfState Yn, WfState acc,
                                                    WfState out, int mode, double a) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    const int k = blockIdx.y;
     if (t >= c.Nx + c.Ny - 1) return;
-    const int i = (t < c.Ny) ? c.Nx - 1 : t - c.Ny;
-    const int j = (t < c.Ny) ? t : c.Ny - 1;
-    const int col = i + c.Nx * j;
-    const int cut = c.cut[col], nzy = c.nzy[col];
-    if (k == cut) return;
-    const long q = wf_q(c, i, j, k);
-    double r[5];
-    if (k < cut) {
-        r[0] = c.u_dead;
-        r[1] = c.v_dead;
-        r[2] = c.w_dead;
-        r[3] = c.T_dead;
-        r[4] = (k < nzy) ? c.Y_dead : 0.0;
-    } else {
-        const WfGlobalAcc acc_g{c, X, D, i, j, k};
-        wf_rhs_node(c, acc_g, k, nzy, r);
-        wf_keep_raw(c, D, col, k, cut, r);
-    }
-    wf_rk_write(mode, a, (unsigned)q, r, (k < c.Nzm) ? 5 : 4, Yn, acc, out);
+    wf_stage_dup_node(c, X, D, Yn, acc, out, mode, a, t, blockIdx.y);
 }
 
 /* Extrapolation giving a zero one-sided z-derivative at the bottom of the fluid
@@ -680,12 +787,22 @@ hipError_t wf_launch_derived(const WfConst &c, const WfState &X, const WfDerived
 hipError_t wf_launch_stage(const WfConst &c, const WfState &X, const WfDerived &D, const WfState &Yn,
                            const WfState &acc, const WfState &out, int mode, double a, hipStream_t s) {
     const int tiles = ((c.Nx + WF_TILE_X - 1) / WF_TILE_X) * ((c.Ny + WF_TILE_Y - 1) / WF_TILE_Y) * (c.Nz - 1);
-    hipLaunchKernelGGL(k_stage, dim3(tiles), dim3(WF_TILE_X, WF_TILE_Y, 1), 0, s, c, X, D, Yn, acc, out, mode, a);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
+#if WF_STAGE_MERGE_DUP
+    const int ndup = ((c.Nx + c.Ny - 1) * (c.Nz - 1) + WF_NT_STAGE - 1) / WF_NT_STAGE;
+#else
+    const int ndup = 0;
     hipLaunchKernelGGL(k_stage_dup, dim3((c.Nx + c.Ny - 1 + 255) / 256, c.Nz - 1), dim3(256), 0, s, c, X, D, Yn, acc,
                        out, mode, a);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+#endif
+    hipLaunchKernelGGL(k_stage, dim3(ndup + tiles), dim3(WF_TILE_X, WF_TILE_Y, 1), 0, s, c, X, D, Yn, acc, out, mode,
+                       a, ndup);
+#if WF_STAGE_MERGE_DUP
+    hipError_t e = hipGetLastError();
+#else
     e = hipGetLastError();
+#endif
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_stage_columns, dim3((c.Nx + 63) / 64, c.Ny), dim3(64), 0, s, c, X, D, Yn, acc, out, mode, a);
     return hipGetLastError();
