@@ -83,6 +83,12 @@ __host__ __device__ constexpr int wf_pairs_for(int n) { return (n <= 300) ? 4 : 
 #define WF_FY_COLS 4
 /* z-levels per lane in the chunked tridiagonal solve and the largest nz */
 #define WF_THOMAS_MAX_CHUNKS 16
+/* half-wave chunks of 4 levels (k_thomas_h) for up to 128 levels: measured
+ * slower (37.5 us against 33.0 us at 256x256x128, its 32-chunk combine is a
+ * longer serial LDS chain), so off by default */
+#ifndef WF_THOMAS_HALF
+#define WF_THOMAS_HALF 0
+#endif
 #define WF_NZ_MAX (16 * WF_THOMAS_MAX_CHUNKS)
 
 struct WfSpectral {

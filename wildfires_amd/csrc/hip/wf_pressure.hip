@@ -553,6 +553,146 @@ __global__ __launch_bounds__(64 * WF_THOMAS_MAX_CHUNKS) void k_thomas(WfSpectral
     }
 }
 
+/* P3 for nz <= 2 WF_THOMAS_MAX_CHUNKS LC (LC = 4: the 128-level grids): the
+ * same chunked solve with each wave taking 32 systems and its two half-waves
+ * two consecutive chunks, so a 16-wave workgroup covers 32 chunks with a
+ * quarter of the registers per lane; two workgroups then fit a CU and one's
+ * loads overlap the other's pivot chain and combine.  The tridiagonal
+ * coefficient tables are staged in LDS (the chunk index, and so k, differs
+ * between the half-waves); the pivots are re-formed with the operations of
+ * k_pivots. */
+template <int LC>
+__global__ __launch_bounds__(64 * WF_THOMAS_MAX_CHUNKS) __attribute__((amdgpu_waves_per_eu(8))) void k_thomas_h(
+    WfSpectral sp) { /* 8 waves per SIMD: <= 64 VGPRs, which this fits without spills */
+    if (sp.ctl->done) return;
+    constexpr int CH = 2 * WF_THOMAS_MAX_CHUNKS;
+    __shared__ double2 sv[CH][32];
+    __shared__ double sg[CH][32];
+    __shared__ double s_ta[CH * LC], s_tc[CH * LC], s_beta[CH * LC];
+    const int nz = sp.nz;
+    for (int t = threadIdx.x; t < nz; t += blockDim.x) {
+        s_ta[t] = (t < nz - 1) ? sp.ta[t] : 0.0;
+        s_tc[t] = (t < nz - 1) ? sp.tc[t] : 0.0;
+        s_beta[t] = sp.beta[t];
+    }
+    const int lane = threadIdx.x & 63, ls = lane & 31;
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int ch = 2 * w + (lane >> 5);
+    const int C = 2 * (int)(blockDim.x >> 6);
+    const int j = blockIdx.x * 32 + ls, r = blockIdx.y;
+    const bool ok = j < sp.ny;
+    const long slab = (long)sp.hx * sp.ldj;
+    const long o = (long)r * sp.ldj + (ok ? j : 0);
+    const int k0 = ch * LC;
+    const double K2 = ok ? sp.kx2[r] + sp.ky2[j] : 0.0;
+    const double c0 = sp.c0a - 0.5 * sp.dz0 * K2 - sp.c0b;
+    __syncthreads();
+    double2 y[LC];
+    double g[LC], ru[LC];
+    /* pivot entering the chunk (a chunk wholly above the top, possible when the
+     * chunk count is odd, has no table entry and no levels) */
+    double u = (ch > 0 && ok && k0 < nz) ? sp.UB[o + ch * slab] : sp.b0;
+#pragma unroll
+    for (int t = 0; t < LC; t++) {
+        const int k = k0 + t;
+        y[t] = make_double2(0.0, 0.0);
+        g[t] = 0.0;
+        ru[t] = 0.0;
+        if (ok && k < nz) {
+            if (k == 0) {
+                double2 f1 = sp.F[o + slab];
+                y[t] = make_double2(0.5 * sp.dz0 * f1.x, 0.5 * sp.dz0 * f1.y);
+                u = sp.b0;
+            } else {
+                double2 d = sp.F[o + k * slab];
+                if (k == nz - 1) {
+                    double2 pt = sp.Ptop[o];
+                    d.x = d.x - pt.x / sp.top_div;
+                    d.y = d.y - pt.y / sp.top_div;
+                }
+                y[t] = d;
+                /* wf_pivot_next with the staged tables */
+                const double ck1 = (k == 1) ? c0 : s_tc[k - 1];
+                const double bk = (k == nz - 1) ? -(K2 + sp.beta_top) : -(K2 + s_beta[k]);
+                const double l = s_ta[k - 1] / u;
+                u = bk - l * ck1;
+                g[t] = l;
+            }
+            ru[t] = 1.0 / u;
+        }
+    }
+    /* forward elimination, entry value 0, coefficient G of the entry value */
+    double2 acc = make_double2(0.0, 0.0);
+    double G = 1.0;
+#pragma unroll
+    for (int t = 0; t < LC; t++) {
+        const int k = k0 + t;
+        if (k == 0) {
+            acc = y[0];
+            G = 0.0;
+        } else {
+            const double m = g[t];
+            acc.x = y[t].x - m * acc.x;
+            acc.y = y[t].y - m * acc.y;
+            G = -m * G;
+            y[t] = acc;
+        }
+        g[t] = G;
+    }
+    sv[ch][ls] = acc;
+    sg[ch][ls] = G;
+    __syncthreads();
+    double2 Y = make_double2(0.0, 0.0);
+    for (int q = 0; q < ch; q++) {
+        const double2 e = sv[q][ls];
+        const double gq = sg[q][ls];
+        Y.x = e.x + gq * Y.x;
+        Y.y = e.y + gq * Y.y;
+    }
+#pragma unroll
+    for (int t = 0; t < LC; t++) {
+        y[t].x = y[t].x + g[t] * Y.x;
+        y[t].y = y[t].y + g[t] * Y.y;
+    }
+    __syncthreads();
+    /* back substitution, entry value x_{k0+LC} = 0, coefficient H */
+    double2 xa = make_double2(0.0, 0.0);
+    double H = 1.0;
+#pragma unroll
+    for (int t = LC - 1; t >= 0; t--) {
+        const int k = k0 + t;
+        if (k >= nz) continue;
+        if (k == nz - 1) {
+            xa.x = y[t].x * ru[t];
+            xa.y = y[t].y * ru[t];
+            H = 0.0;
+        } else {
+            const double ck = (k == 0) ? c0 : s_tc[k];
+            xa.x = (y[t].x - ck * xa.x) * ru[t];
+            xa.y = (y[t].y - ck * xa.y) * ru[t];
+            H = (-ck * ru[t]) * H;
+        }
+        y[t] = xa;
+        g[t] = H;
+    }
+    sv[ch][ls] = xa;
+    sg[ch][ls] = H;
+    __syncthreads();
+    double2 X = make_double2(0.0, 0.0);
+    for (int q = C - 1; q > ch; q--) {
+        const double2 e = sv[q][ls];
+        const double hq = sg[q][ls];
+        X.x = e.x + hq * X.x;
+        X.y = e.y + hq * X.y;
+    }
+    if (!ok) return;
+#pragma unroll
+    for (int t = 0; t < LC; t++) {
+        const int k = k0 + t;
+        if (k < nz) sp.F[o + k * slab] = make_double2(y[t].x + g[t] * X.x, y[t].y + g[t] * X.y);
+    }
+}
+
 /* P5: inverse x-transform of one row-pair group of slab k (0..nz-1):
  * p = Re(.)/(nx ny) (multiplied by the reciprocal) on unique nodes, duplicate
  * planes i = Nx-1 and j = Ny-1 set from their images [Q6], and max
@@ -884,6 +1024,11 @@ hipError_t wf_launch_pivots(const WfSpectral &sp, double *UB, hipStream_t s) {
 hipError_t wf_launch_thomas(const WfSpectral &sp, hipStream_t s) {
     dim3 grd((sp.ny + 63) / 64, sp.hx, 1);
     const int chunks = (sp.nz + sp.thomas_lc - 1) / sp.thomas_lc;
+    if (sp.thomas_lc == 4) {
+        if (chunks > 2 * WF_THOMAS_MAX_CHUNKS) return hipErrorInvalidValue;
+        hipLaunchKernelGGL(k_thomas_h<4>, dim3((sp.ny + 31) / 32, sp.hx, 1), dim3(64 * ((chunks + 1) / 2)), 0, s, sp);
+        return hipGetLastError();
+    }
     if (chunks > WF_THOMAS_MAX_CHUNKS) return hipErrorInvalidValue;
     if (sp.thomas_lc == 8)
         hipLaunchKernelGGL(k_thomas<8>, grd, dim3(64 * chunks), 0, s, sp);

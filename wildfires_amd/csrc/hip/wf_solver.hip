@@ -414,7 +414,10 @@ static int wf_create_impl(wf_solver *s, const Parameters *P) {
     WF_HIP(hipMemset(s->F, 0, sizeof(double2) * nspec));
     WF_HIP(hipMalloc((void **)&s->Ptop, sizeof(double2) * (long)sp.hx * sp.ldj));
     WF_HIP(hipMemset(s->Ptop, 0, sizeof(double2) * (long)sp.hx * sp.ldj));
-    sp.thomas_lc = (sp.nz <= 8 * WF_THOMAS_MAX_CHUNKS) ? 8 : 16;
+    /* 4 levels per half-wave chunk (k_thomas_h) up to 128 levels, else whole-wave chunks of 8 or 16 */
+    sp.thomas_lc = (WF_THOMAS_HALF && sp.nz <= 4 * 2 * WF_THOMAS_MAX_CHUNKS) ? 4
+                   : (sp.nz <= 8 * WF_THOMAS_MAX_CHUNKS)                       ? 8
+                                                                               : 16;
     const int nchunks = (sp.nz + sp.thomas_lc - 1) / sp.thomas_lc;
     WF_HIP(hipMalloc((void **)&s->UB, sizeof(double) * (long)nchunks * sp.hx * sp.ldj));
     WF_HIP(hipMemset(s->UB, 0, sizeof(double) * (long)nchunks * sp.hx * sp.ldj));
