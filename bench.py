@@ -152,10 +152,58 @@ def run(args):
                           "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(step_achieved / HBM_PEAK_GBS, 4)},
         "kernels": kernels,
     }
+    if not args.no_closure:
+        result["learned_closure"] = closure_leg(solver, params, y0, p0, N, args)
     solver.close()
     if not args.no_cpu_baseline:
         result["cpu_baseline"] = cpu_baseline(params, y0, p0)
     return result
+
+
+def closure_leg(solver, params, y0, p0, N, args):
+    """The same workload and step count with the learned subgrid closure on
+    (BF16 MFMA network in the derived pass, FP64 elsewhere): Mcell-steps/s,
+    and the derived pass's mean launch time next to the plain one."""
+    import numpy as np
+    from wildfires_amd import closure
+    path = os.path.join(ROOT, "wildfires_amd", "data", "closure.npz")
+    if os.path.exists(path):
+        block, weights = closure.load(path), "trained (wildfires_amd/data/closure.npz)"
+    else:
+        block, weights = closure.random_block(0, out_scale=1e-3), "untrained (seeded random, full input range)"
+    solver.set_state(y0, p0)
+    solver.set_closure(block)
+    n = 1
+    if args.warmup:
+        solver.step(n, args.warmup)
+        n += args.warmup
+    s0 = solver.stats()
+    api_sync()
+    t0 = time.perf_counter()
+    solver.step(n, args.steps)
+    api_sync()
+    elapsed = time.perf_counter() - t0
+    n += args.steps
+    s1 = solver.stats()
+    solver.profile(True)
+    solver.step(n, 1)
+    times = solver.kernel_times()
+    solver.profile(False)
+    y, _ = solver.get_state()
+    solver.set_closure(None)
+    rate = N * args.steps / elapsed / 1e6
+    ms, cnt = times.get("derived", (0.0, 0))
+    log("closure leg: %d steps in %.3f s: %.1f Mcell-steps/s" % (args.steps, elapsed, rate))
+    return {"value": round(rate, 3), "unit": "Mcell-steps/s", "ms_per_step": round(1000.0 * elapsed / args.steps, 4),
+            "dtype": "f64 state, bf16 x bf16 -> f32 MFMA closure network", "weights": weights,
+            "pressure_solves_per_step": round((s1["pressure_solves"] - s0["pressure_solves"]) / args.steps, 3),
+            "derived_mean_us": round(1000.0 * ms / cnt, 2) if cnt else None,
+            "state_finite": bool(np.isfinite(y).all())}
+
+
+def api_sync():
+    from wildfires_amd import api
+    api.synchronize()
 
 
 def cpu_baseline(params, y0, p0):
@@ -339,6 +387,7 @@ def main():
     ap.add_argument("--profile-steps", type=int, default=2)
     ap.add_argument("--workload", default="hill256", choices=["hill256", "flat64"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-closure", action="store_true", help="skip the learned-closure leg")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run device-to-device copies of BYTES (counter scale factor under rocprofv3 --pmc)")
     ap.add_argument("--rocprof-summary", metavar="DIR",

@@ -361,6 +361,57 @@ int wf_solver_kernel_times(const wf_solver *solver, int kind, double *total_ms, 
 /* Device synchronisation for callers that time whole steps. */
 int wf_device_synchronize(void);
 
+/* ---- Learned subgrid closure ------------------------------------------------
+ * No counterpart in the reference: an MLP 12 -> 64 -> 64 -> 1 (hidden layers
+ * with the sigmoid form of GELU, h / (1 + exp(-1.702 h)); softplus output)
+ * gives nu_t per node in place of the Smagorinsky
+ * mu_sgs = rho (C_s Delta)^2 |S| of src/cu/pde.cu:745 and of the van Driest
+ * damping of src/cu/turbulence.cu:345-356.  It runs with BF16 weights and
+ * activations and FP32 accumulation.  Nodes whose inputs leave the training
+ * range [lo, hi], or whose output is not finite, keep plain (damped)
+ * Smagorinsky.
+ *
+ * Inputs per node: ux uy uz vx vy vz wx wy wz (central differences of the
+ * stage input), |grad T|, Delta = (dx dy dz_k)^(1/3), Y.
+ *
+ * Parameter block, WF_CLOSURE_NPARAMS floats in this order:
+ *   lo[12], hi[12]         training range of the raw inputs
+ *   mean[12], inv_std[12]  normalisation x = (f - mean) * inv_std
+ *   W1[64][12], b1[64], W2[64][64], b2[64], w3[64], b3
+ *   out_scale              nu_t = out_scale * softplus(w3 . a2 + b3)   (m^2/s) */
+#define WF_CLOSURE_NIN 12
+#define WF_CLOSURE_NHIDDEN 64
+#define WF_CLOSURE_NPARAMS (4 * 12 + 64 * 12 + 64 + 64 * 64 + 64 + 64 + 1 + 1)
+
+/* Enables the closure for the solver's later steps (params != NULL, n ==
+ * WF_CLOSURE_NPARAMS, all finite) or disables it (params == NULL). */
+int wf_solver_set_closure(wf_solver *solver, const float *params, long n);
+/* The network alone on n rows of raw inputs (features: n x 12, row-major):
+ * nu_t[r], and learned[r] = 1 when row r is in the training range with a
+ * finite output (else the solver would use Smagorinsky there). */
+int wf_closure_eval(const float *params, long nparams, const double *features, long n, double *nu_t, int *learned);
+
+/* Training of the closure on the GPU with its own BF16 MFMA forward and
+ * backward kernels (FP32 accumulation) and Adam on FP32 master weights.
+ * Loss over a batch of rows: L = sum_r (softplus(z_r) - nu_r/out_scale)^2 / batch.
+ * The normalisation, training range and out_scale of the initial parameters
+ * stay fixed; W1, b1, W2, b2, w3, b3 are trained. */
+typedef struct wf_closure_trainer wf_closure_trainer;
+/* Uploads n rows of raw inputs (n x 12, row-major) and target nu_t (n). */
+int wf_closure_trainer_create(const float *params, long nparams, const double *features, const double *nu_target,
+                              long n, wf_closure_trainer **trainer);
+/* Loss weight * sum (softplus(z) - t)^2 over rows [offset, offset + count) at the
+ * current parameters, and its gradient (a WF_CLOSURE_NPARAMS array; entries
+ * that are not trained are 0). */
+int wf_closure_trainer_gradient(wf_closure_trainer *trainer, long offset, long count, double weight, float *grad,
+                                double *loss);
+/* steps Adam steps (learning rate lr) over consecutive batches of the rows;
+ * *last_loss (may be NULL) gets the last batch's loss. */
+int wf_closure_trainer_train(wf_closure_trainer *trainer, int steps, long batch, double lr, double *last_loss);
+/* Current parameter block (WF_CLOSURE_NPARAMS floats). */
+int wf_closure_trainer_params(wf_closure_trainer *trainer, float *params);
+int wf_closure_trainer_destroy(wf_closure_trainer *trainer);
+
 #ifdef __cplusplus
 }
 #endif

@@ -15,6 +15,7 @@ _lib = None
 
 DBL_P = ctypes.POINTER(ctypes.c_double)
 INT_P = ctypes.POINTER(ctypes.c_int)
+FLT_P = ctypes.POINTER(ctypes.c_float)
 
 
 def load(rebuild=False):
@@ -44,6 +45,9 @@ def load(rebuild=False):
         "oracle_tridiagonal_coefficients": (None, [P, ctypes.c_int, ctypes.c_int, DBL_P, DBL_P, DBL_P]),
         "oracle_thomas": (None, [ctypes.c_int, DBL_P, DBL_P, DBL_P, DBL_P, DBL_P]),
         "oracle_threads": (ctypes.c_int, []),
+        "oracle_set_closure": (None, [FLT_P]),
+        "oracle_closure_eval": (None, [FLT_P, DBL_P, ctypes.c_long, DBL_P, INT_P]),
+        "oracle_closure_gradient": (None, [FLT_P, DBL_P, DBL_P, ctypes.c_long, ctypes.c_double, FLT_P, DBL_P]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -177,3 +181,42 @@ def thomas(a, b, c, d):
 
 def threads():
     return load().oracle_threads()
+
+
+_closure_keep = None
+
+
+def set_closure(closure_params):
+    """Learned closure in every later oracle RHS (None: Smagorinsky).  The
+    oracle keeps a pointer: the array is held here until replaced."""
+    global _closure_keep
+    lib = load()
+    if closure_params is None:
+        _closure_keep = None
+        lib.oracle_set_closure(None)
+        return
+    _closure_keep = np.ascontiguousarray(closure_params, dtype=np.float32)
+    lib.oracle_set_closure(_closure_keep.ctypes.data_as(FLT_P))
+
+
+def closure_eval(closure_params, features):
+    prm = np.ascontiguousarray(closure_params, dtype=np.float32)
+    f = np.ascontiguousarray(features, dtype=np.float64).reshape(-1, 12)
+    nu = np.zeros(f.shape[0])
+    learned = np.zeros(f.shape[0], dtype=np.int32)
+    load().oracle_closure_eval(prm.ctypes.data_as(FLT_P), _dp(f), f.shape[0], _dp(nu), _ip(learned))
+    return nu, learned
+
+
+def closure_gradient(closure_params, features, nu_target, weight=None):
+    """Loss weight * sum (softplus(z) - nu/out_scale)^2 and its gradient
+    (parameter-block-shaped float32; weight defaults to 1/n)."""
+    prm = np.ascontiguousarray(closure_params, dtype=np.float32)
+    f = np.ascontiguousarray(features, dtype=np.float64).reshape(-1, 12)
+    t = np.ascontiguousarray(nu_target, dtype=np.float64)
+    w = 1.0 / len(t) if weight is None else weight
+    grad = np.zeros(prm.size, dtype=np.float32)
+    loss = ctypes.c_double(0.0)
+    load().oracle_closure_gradient(prm.ctypes.data_as(FLT_P), _dp(f), _dp(t), len(t), w, grad.ctypes.data_as(FLT_P),
+                                   ctypes.byref(loss))
+    return grad, loss.value

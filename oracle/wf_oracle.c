@@ -17,6 +17,7 @@
 #include <omp.h>
 #include <stdlib.h>
 #include <string.h>
+#include <stdint.h>
 
 #define IDX(i, j, k, Nx, Ny, Nz) ((long)(k) + (long)(Nz) * ((long)(j) + (long)(Ny) * (long)(i)))
 #define FIDX(i, j, k, Nx, Ny) ((long)(j) + (long)(Ny) * (long)(i) + (long)(Nx) * (long)(Ny) * (long)(k))
@@ -113,6 +114,151 @@ static inline double d2_mid(double fm1, double f0, double fp1, double hm1, doubl
 }
 
 /* ------------------------------------------------------------------ RHS ---- */
+
+/* ---- learned subgrid closure (no reference counterpart) -------------------
+ * Restates the network of wildfires_amd/csrc/hip/wf_closure.h with its
+ * rounding points: bf16 inputs, weights and activations, FP32 bias, GELU and
+ * softplus.  The dot products are accumulated exactly enough (double) and
+ * rounded to FP32 once, where the GPU accumulates in FP32 inside MFMA: the
+ * two differ by FP32 accumulation error only.  Parameter layout: the
+ * WF_CLOSURE_NPARAMS block of include/wildfires.h.  One closure at a time per
+ * process (set by oracle_set_closure). */
+#define O_CL_LO 0
+#define O_CL_HI 12
+#define O_CL_MEAN 24
+#define O_CL_ISTD 36
+#define O_CL_W1 48
+#define O_CL_B1 (O_CL_W1 + 64 * 12)
+#define O_CL_W2 (O_CL_B1 + 64)
+#define O_CL_B2 (O_CL_W2 + 64 * 64)
+#define O_CL_W3 (O_CL_B2 + 64)
+#define O_CL_B3 (O_CL_W3 + 64)
+#define O_CL_SCALE (O_CL_B3 + 1)
+
+static const float *o_closure = NULL;
+
+void oracle_set_closure(const float *params) { o_closure = params; }
+
+/* round to the nearest bf16 (ties to even), returned as a float */
+static float o_bf16(float f) {
+    uint32_t u;
+    memcpy(&u, &f, 4);
+    if ((u & 0x7f800000u) == 0x7f800000u) return f;
+    u += 0x7fffu + ((u >> 16) & 1u);
+    u &= 0xffff0000u;
+    memcpy(&f, &u, 4);
+    return f;
+}
+static float o_gelu(float h) { return h / (1.0f + expf(-1.702f * h)); } /* sigmoid form of GELU */
+static float o_softplus(float z) { return (z > 20.0f) ? z : log1pf(expf(z)); }
+
+/* nu_t of one node and whether its inputs are in the training range with a
+ * finite output; f = ux uy uz vx vy vz wx wy wz |grad T| Delta Y */
+static int o_closure_nu(const float *prm, const double f[12], float *nu_out) {
+    int in = 1;
+    float x[12], a1[64], a2[64];
+    for (int i = 0; i < 12; i++) {
+        in = in && (f[i] >= (double)prm[O_CL_LO + i]) && (f[i] <= (double)prm[O_CL_HI + i]);
+        x[i] = o_bf16((float)((f[i] - (double)prm[O_CL_MEAN + i]) * (double)prm[O_CL_ISTD + i]));
+    }
+    for (int n = 0; n < 64; n++) {
+        double acc = 0.0;
+        for (int i = 0; i < 12; i++) acc += (double)o_bf16(prm[O_CL_W1 + n * 12 + i]) * (double)x[i];
+        a1[n] = o_bf16(o_gelu((float)acc + prm[O_CL_B1 + n]));
+    }
+    for (int n = 0; n < 64; n++) {
+        double acc = 0.0;
+        for (int i = 0; i < 64; i++) acc += (double)o_bf16(prm[O_CL_W2 + n * 64 + i]) * (double)a1[i];
+        a2[n] = o_bf16(o_gelu((float)acc + prm[O_CL_B2 + n]));
+    }
+    double acc = 0.0;
+    for (int i = 0; i < 64; i++) acc += (double)o_bf16(prm[O_CL_W3 + i]) * (double)a2[i];
+    const float z = (float)acc + prm[O_CL_B3];
+    const float nu = prm[O_CL_SCALE] * o_softplus(z);
+    *nu_out = nu;
+    return in && nu >= 0.0f && nu < 3.0e38f;
+}
+
+void oracle_closure_eval(const float *params, const double *features, long n, double *nu_t, int *learned) {
+    for (long r = 0; r < n; r++) {
+        float nu;
+        learned[r] = o_closure_nu(params, features + 12 * r, &nu);
+        nu_t[r] = (double)nu;
+    }
+}
+
+static float o_sigmoid(float v) { return 1.0f / (1.0f + expf(-v)); }
+/* derivative of the sigmoid form of GELU */
+static float o_gelu_d(float h) {
+    const float s = 1.0f / (1.0f + expf(-1.702f * h));
+    return s + 1.702f * h * s * (1.0f - s);
+}
+
+/* Training loss L = w * sum_r (softplus(z_r) - nu_r / out_scale)^2 over n rows
+ * and its gradient with respect to W1, b1, W2, b2, w3, b3 (placed at their
+ * offsets of the parameter block; the other entries are 0), restating the
+ * GPU trainer's rounding points: the forward of o_closure_nu, then
+ *   dz  = 2 w (yhat - t) sigmoid(z)                      (FP32)
+ *   dh2 = dz w3 gelu'(h2),  dh1 = (W2^T bf16(dh2)) gelu'(h1)  (FP32)
+ *   dW2 = sum bf16(dh2) a1^T,  dW1 = sum bf16(dh1) x^T   (bf16 products)
+ *   db2, db1, dw3, db3 from the FP32 deltas.
+ * Sums are accumulated in double and rounded to FP32 once. */
+void oracle_closure_gradient(const float *prm, const double *features, const double *nu_target, long n,
+                             double weight, float *grad, double *loss) {
+    static const int NP = O_CL_SCALE + 1;
+    double *g = (double *)calloc(NP, sizeof(double));
+    double L = 0.0;
+    const float w = (float)weight;
+    for (long r = 0; r < n; r++) {
+        const double *f = features + 12 * r;
+        float x[12], h1[64], a1[64], h2[64], a2[64], dh2[64], dh1[64];
+        for (int i = 0; i < 12; i++)
+            x[i] = o_bf16((float)((f[i] - (double)prm[O_CL_MEAN + i]) * (double)prm[O_CL_ISTD + i]));
+        for (int q = 0; q < 64; q++) {
+            double acc = 0.0;
+            for (int i = 0; i < 12; i++) acc += (double)o_bf16(prm[O_CL_W1 + q * 12 + i]) * (double)x[i];
+            h1[q] = (float)acc + prm[O_CL_B1 + q];
+            a1[q] = o_bf16(o_gelu(h1[q]));
+        }
+        for (int q = 0; q < 64; q++) {
+            double acc = 0.0;
+            for (int i = 0; i < 64; i++) acc += (double)o_bf16(prm[O_CL_W2 + q * 64 + i]) * (double)a1[i];
+            h2[q] = (float)acc + prm[O_CL_B2 + q];
+            a2[q] = o_bf16(o_gelu(h2[q]));
+        }
+        double acc = 0.0;
+        for (int i = 0; i < 64; i++) acc += (double)o_bf16(prm[O_CL_W3 + i]) * (double)a2[i];
+        const float z = (float)acc + prm[O_CL_B3];
+        const float yhat = o_softplus(z);
+        const float t = (float)(nu_target[r] / (double)prm[O_CL_SCALE]);
+        const float e = yhat - t;
+        L += (double)w * (double)e * (double)e;
+        const float dz = 2.0f * w * e * o_sigmoid(z);
+        g[O_CL_B3] += dz;
+        for (int q = 0; q < 64; q++) {
+            g[O_CL_W3 + q] += (double)dz * (double)a2[q];
+            dh2[q] = dz * o_bf16(prm[O_CL_W3 + q]) * o_gelu_d(h2[q]);
+            g[O_CL_B2 + q] += dh2[q];
+            const float d = o_bf16(dh2[q]);
+            for (int k = 0; k < 64; k++) g[O_CL_W2 + q * 64 + k] += (double)d * (double)a1[k];
+        }
+        for (int k = 0; k < 64; k++) {
+            double da = 0.0;
+            for (int q = 0; q < 64; q++) da += (double)o_bf16(dh2[q]) * (double)o_bf16(prm[O_CL_W2 + q * 64 + k]);
+            dh1[k] = (float)da * o_gelu_d(h1[k]);
+            g[O_CL_B1 + k] += dh1[k];
+            const float d = o_bf16(dh1[k]);
+            for (int i = 0; i < 12; i++) g[O_CL_W1 + k * 12 + i] += (double)d * (double)x[i];
+        }
+    }
+    for (int q = 0; q < NP; q++) grad[q] = (float)g[q];
+    *loss = L;
+    free(g);
+}
+
+/* mu_sgs the turbulence pass uses at a point: a learned value (stored
+ * negative) as it is, a Smagorinsky value (>= 0) damped by f^2 */
+static double o_damped(double mu, double f) { return (mu < 0.0) ? -mu : mu * (f * f); }
 
 /* Per-cell right-hand side (src/cu/pde.cu:139-787).  [Q1] the loop-carried h_c
  * of src/cu/pde.cu:698 is evaluated per cell from the parameter value;
@@ -295,6 +441,12 @@ void oracle_rhs(const Parameters *P, const double *R_old, double *R_new, double 
         double w_RHS = nu * lap_w - (uwx + vwy + wwz) + F_z;
         double cdelta = C_s * pow(dx * dy * dz_k, 1.0 / 3.0);
         double mu_sgs = rho_ijk * pow(cdelta, 2.0) * mod_S;
+        if (o_closure) { /* learned nu_t, stored as -rho nu_t (not damped) */
+            const double f[12] = {ux, uy, uz, vx, vy, vz, wx, wy, wz, sqrt(Tx * Tx + Ty * Ty + Tz * Tz),
+                                  pow(dx * dy * dz_k, 1.0 / 3.0), Y_ijk};
+            float nu;
+            if (o_closure_nu(o_closure, f, &nu)) mu_sgs = -(rho_ijk * (double)nu);
+        }
         long q3 = IDX(i, j, k, Nx, Ny, Nz);
         R_turb[ti.rho + q3] = rho_ijk;
         R_turb[ti.Tx + q3] = Tx;
@@ -385,11 +537,11 @@ void oracle_turbulence(const Parameters *P, const double *R_turb, double *R_new)
         double fw_im = fw_damp(z_ibm[cxm], ut_im, nu);
         double fw_jp = fw_damp(z_ibm[cyp], ut_jp, nu);
         double fw_jm = fw_damp(z_ibm[cym], ut_jm, nu);
-        mu_sgs_ijk *= fw * fw;
-        mu_im *= fw_im * fw_im;
-        mu_ip *= fw_ip * fw_ip;
-        mu_jm *= fw_jm * fw_jm;
-        mu_jp *= fw_jp * fw_jp;
+        mu_sgs_ijk = o_damped(mu_sgs_ijk, fw);
+        mu_im = o_damped(mu_im, fw_im);
+        mu_ip = o_damped(mu_ip, fw_ip);
+        mu_jm = o_damped(mu_jm, fw_jm);
+        mu_jp = o_damped(mu_jp, fw_jp);
         if (k > 0) {
             long cz = IDX(i, j, k - 1, Nx, Ny, Nz);
             double f = fw_damp(z_ibm[cz], ut, nu);
@@ -398,7 +550,7 @@ void oracle_turbulence(const Parameters *P, const double *R_turb, double *R_new)
             S_32_km1 = R_turb[ti.S_32 + cz];
             S_33_km1 = R_turb[ti.S_33 + cz];
             dU_km1 = R_turb[ti.div_U + cz];
-            mu_km1 *= f * f;
+            mu_km1 = o_damped(mu_km1, f);
         }
         if (k < Nz - 1) {
             long cz = IDX(i, j, k + 1, Nx, Ny, Nz);
@@ -408,7 +560,7 @@ void oracle_turbulence(const Parameters *P, const double *R_turb, double *R_new)
             S_32_kp1 = R_turb[ti.S_32 + cz];
             S_33_kp1 = R_turb[ti.S_33 + cz];
             dU_kp1 = R_turb[ti.div_U + cz];
-            mu_kp1 *= f * f;
+            mu_kp1 = o_damped(mu_kp1, f);
         }
         double mu_x = (mu_ip - mu_im) / (2 * dx);
         double mu_y = (mu_jp - mu_jm) / (2 * dy);
@@ -425,7 +577,7 @@ void oracle_turbulence(const Parameters *P, const double *R_turb, double *R_new)
             double h0 = z[k + 1] - z[k], h1 = z[k + 2] - z[k + 1];
             long cz = IDX(i, j, k + 2, Nx, Ny, Nz);
             double f = fw_damp(z_ibm[cz], ut, nu);
-            double mu_kp2 = R_turb[ti.mu_sgs + cz] * (f * f);
+            double mu_kp2 = o_damped(R_turb[ti.mu_sgs + cz], f);
             mu_z = d1_fwd(mu_sgs_ijk, mu_kp1, mu_kp2, h0, h1);
             S_31_z = d1_fwd(S_31, S_31_kp1, R_turb[ti.S_31 + cz], h0, h1);
             S_32_z = d1_fwd(S_32, S_32_kp1, R_turb[ti.S_32 + cz], h0, h1);
@@ -435,7 +587,7 @@ void oracle_turbulence(const Parameters *P, const double *R_turb, double *R_new)
             double hm1 = z[k] - z[k - 1], hm2 = z[k - 1] - z[k - 2];
             long cz = IDX(i, j, k - 2, Nx, Ny, Nz);
             double f = fw_damp(z_ibm[cz], ut, nu);
-            double mu_km2 = R_turb[ti.mu_sgs + cz] * (f * f);
+            double mu_km2 = o_damped(R_turb[ti.mu_sgs + cz], f);
             mu_z = d1_bwd(mu_km2, mu_km1, mu_sgs_ijk, hm2, hm1);
             S_31_z = d1_bwd(R_turb[ti.S_31 + cz], S_31_km1, S_31, hm2, hm1);
             S_32_z = d1_bwd(R_turb[ti.S_32 + cz], S_32_km1, S_32, hm2, hm1);

@@ -57,6 +57,18 @@ int oracle_run(oracle_ctx *ctx, double *y_n, double *p, int step0, int n_steps);
 /* Sum of pressure solves over the steps run so far. */
 long oracle_pressure_solves(const oracle_ctx *ctx);
 
+/* Learned subgrid closure (restating wildfires_amd/csrc/hip/wf_closure.h):
+ * params is the WF_CLOSURE_NPARAMS block of include/wildfires.h, kept by
+ * pointer (NULL: plain Smagorinsky).  One closure per process. */
+void oracle_set_closure(const float *params);
+/* The network alone on n rows of the 12 raw inputs (see wf_solver_set_closure). */
+void oracle_closure_eval(const float *params, const double *features, long n, double *nu_t, int *learned);
+/* Training loss weight * sum (softplus(z) - nu_target/out_scale)^2 over n rows
+ * and its gradient (a parameter-block-shaped float array; the normalisation,
+ * range and out_scale entries are 0), restating the GPU trainer. */
+void oracle_closure_gradient(const float *params, const double *features, const double *nu_target, long n,
+                             double weight, float *grad, double *loss);
+
 /* In-place complex DFT of length n over interleaved (re,im) doubles at the given
  * element stride; sign -1 forward, +1 inverse, no normalisation. */
 void oracle_fft(double *data, int n, int stride, int sign);

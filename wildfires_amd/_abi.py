@@ -70,6 +70,7 @@ def _build_types():
 Parameters, STRUCTS = _build_types()
 PARAMS_P = ctypes.POINTER(Parameters)
 DBL_P = ctypes.POINTER(ctypes.c_double)
+FLT_P = ctypes.POINTER(ctypes.c_float)
 INT_P = ctypes.POINTER(ctypes.c_int)
 
 # (name, restype, argtypes) of every exported entry point of include/wildfires.h
@@ -118,6 +119,15 @@ EXPORTS = [
     ("wf_solver_kernel_times", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, DBL_P, ctypes.POINTER(ctypes.c_long)]),
     ("wf_solver_last_pressure", ctypes.c_int, [ctypes.c_void_p, DBL_P, INT_P]),
     ("wf_device_synchronize", ctypes.c_int, []),
+    ("wf_solver_set_closure", ctypes.c_int, [ctypes.c_void_p, FLT_P, ctypes.c_long]),
+    ("wf_closure_eval", ctypes.c_int, [FLT_P, ctypes.c_long, DBL_P, ctypes.c_long, DBL_P, INT_P]),
+    ("wf_closure_trainer_create", ctypes.c_int, [FLT_P, ctypes.c_long, DBL_P, DBL_P, ctypes.c_long,
+                                                 ctypes.POINTER(ctypes.c_void_p)]),
+    ("wf_closure_trainer_gradient", ctypes.c_int, [ctypes.c_void_p, ctypes.c_long, ctypes.c_long, ctypes.c_double,
+                                                   FLT_P, DBL_P]),
+    ("wf_closure_trainer_train", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_long, ctypes.c_double, DBL_P]),
+    ("wf_closure_trainer_params", ctypes.c_int, [ctypes.c_void_p, FLT_P]),
+    ("wf_closure_trainer_destroy", ctypes.c_int, [ctypes.c_void_p]),
 ]
 
 _lib = None

@@ -209,6 +209,16 @@ class Solver:
     def step(self, step0, n_steps):
         _check(lib().wf_solver_step(self.handle, int(step0), int(n_steps)))
 
+    def set_closure(self, closure_params):
+        """Learned subgrid closure for the following steps (a
+        WF_CLOSURE_NPARAMS float32 block, see closure.py), or None for plain
+        Smagorinsky."""
+        if closure_params is None:
+            _check(lib().wf_solver_set_closure(self.handle, None, 0))
+            return
+        prm = np.ascontiguousarray(closure_params, dtype=np.float32)
+        _check(lib().wf_solver_set_closure(self.handle, prm.ctypes.data_as(_abi.FLT_P), prm.size))
+
     def phi(self, R_old):
         R_old = np.ascontiguousarray(R_old, dtype=np.float64)
         R_new = np.zeros_like(R_old)
@@ -253,3 +263,64 @@ class Solver:
 
 def synchronize():
     _check(lib().wf_device_synchronize())
+
+
+def closure_eval(closure_params, features):
+    """The closure network on rows of its 12 raw inputs (wf_closure_eval):
+    returns nu_t (float64) and the learned flags (1 where the row is in the
+    training range with a finite output)."""
+    prm = np.ascontiguousarray(closure_params, dtype=np.float32)
+    f = np.ascontiguousarray(features, dtype=np.float64).reshape(-1, 12)
+    nu = np.zeros(f.shape[0])
+    learned = np.zeros(f.shape[0], dtype=np.int32)
+    _check(lib().wf_closure_eval(prm.ctypes.data_as(_abi.FLT_P), prm.size, _dp(f), f.shape[0], _dp(nu),
+                                 learned.ctypes.data_as(_abi.INT_P)))
+    return nu, learned
+
+
+class ClosureTrainer:
+    """GPU training of the closure network (wf_closure_trainer_*): rows of raw
+    inputs (n x 12) with target nu_t; the normalisation, range and out_scale
+    of the initial block stay fixed."""
+
+    def __init__(self, closure_params, features, nu_target):
+        prm = np.ascontiguousarray(closure_params, dtype=np.float32)
+        self._f = np.ascontiguousarray(features, dtype=np.float64).reshape(-1, 12)
+        self._t = np.ascontiguousarray(nu_target, dtype=np.float64)
+        if len(self._t) != self._f.shape[0]:
+            raise ValueError("features and targets differ in length")
+        self.n = len(self._t)
+        self.handle = ctypes.c_void_p()
+        _check(lib().wf_closure_trainer_create(prm.ctypes.data_as(_abi.FLT_P), prm.size, _dp(self._f), _dp(self._t),
+                                               self.n, ctypes.byref(self.handle)))
+
+    def gradient(self, offset, count, weight):
+        g = np.zeros(WF_CLOSURE_NPARAMS, dtype=np.float32)
+        loss = ctypes.c_double(0.0)
+        _check(lib().wf_closure_trainer_gradient(self.handle, int(offset), int(count), float(weight),
+                                                 g.ctypes.data_as(_abi.FLT_P), ctypes.byref(loss)))
+        return g, loss.value
+
+    def train(self, steps, batch, lr):
+        loss = ctypes.c_double(0.0)
+        _check(lib().wf_closure_trainer_train(self.handle, int(steps), int(batch), float(lr), ctypes.byref(loss)))
+        return loss.value
+
+    def params(self):
+        p = np.zeros(WF_CLOSURE_NPARAMS, dtype=np.float32)
+        _check(lib().wf_closure_trainer_params(self.handle, p.ctypes.data_as(_abi.FLT_P)))
+        return p
+
+    def close(self):
+        if self.handle:
+            lib().wf_closure_trainer_destroy(self.handle)
+            self.handle = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+WF_CLOSURE_NPARAMS = 5106

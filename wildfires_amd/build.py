@@ -16,7 +16,7 @@ LIB = os.path.join(PKG, "libwildfires_amd.so")
 EXE = os.path.join(PKG, "wildfires_amd")
 
 HOST_SOURCES = ["params.c", "terrain.c", "init.c", "output.c", "solver.c"]
-HIP_SOURCES = ["wf_stage.hip", "wf_pressure.hip", "wf_project.hip", "wf_solver.hip"]
+HIP_SOURCES = ["wf_stage.hip", "wf_pressure.hip", "wf_project.hip", "wf_solver.hip", "wf_closure_train.hip"]
 
 # FP64 kernels are compiled without FMA contraction (expressions evaluate in
 # the written order) and without packed FP32 instructions.

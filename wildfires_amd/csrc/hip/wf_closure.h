@@ -1,0 +1,194 @@
+/*
+ * wf_closure.h -- learned subgrid closure: MLP 12 -> 64 -> 64 -> 1 (GELU,
+ * softplus output) giving nu_t per node, evaluated with BF16 MFMA and FP32
+ * accumulation, 64 nodes (one wavefront) at a time.
+ *
+ * It has no counterpart in the reference.  It replaces the Smagorinsky
+ * mu_sgs = rho (C_s Delta)^2 |S| of src/cu/pde.cu:745 and, for the nodes it
+ * handles, the van Driest damping of src/cu/turbulence.cu:345-356: a learned
+ * node stores mu_sgs = -rho nu_t (the sign tells the stage pass not to damp
+ * it).  Nodes whose inputs leave the training range, or whose output is not
+ * finite, keep the Smagorinsky value (positive, damped as usual).
+ *
+ * Inputs (raw, FP64 from the state): ux uy uz vx vy vz wx wy wz, |grad T|,
+ * Delta = (dx dy dz_k)^(1/3), Y.  Rounding points, which the oracle repeats:
+ *   x_i  = bf16(float((f_i - mean_i) * inv_std_i))      normalised input
+ *   h1   = b1 + W1 x        (FP32 accumulation of bf16 x bf16 products)
+ *   a1   = bf16(gelu(h1)),  gelu(h) = h / (1 + exp(-1.702 h))  in FP32 (the
+ *          sigmoid form of GELU, with the hardware exp and reciprocal)
+ *   h2   = b2 + W2 a1,  a2 = bf16(gelu(h2))
+ *   z    = b3 + w3 . a2
+ *   nu_t = out_scale * softplus(z),  softplus(z) = z > 20 ? z : log1p(exp(z))
+ * Weights are rounded to bf16 (nearest even) once, when they are set.
+ *
+ * MFMA v_mfma_f32_16x16x32_bf16 operand layout (lane l of 64):
+ *   A (16 x 32, rows = nodes):   A[l % 16][8 (l / 16) + j],  j = 0..7
+ *   B (32 x 16, cols = outputs): B[8 (l / 16) + j][l % 16]
+ *   D (16 x 16, FP32):           D[4 (l / 16) + r][l % 16],  r = 0..3
+ */
+#ifndef WF_CLOSURE_H
+#define WF_CLOSURE_H
+
+#include "wf_common.h"
+
+#define WF_CL_NIN 12
+#define WF_CL_NH 64
+/* parameter block (floats), the layout of wf_solver_set_closure */
+#define WF_CL_LO 0
+#define WF_CL_HI (WF_CL_LO + WF_CL_NIN)
+#define WF_CL_MEAN (WF_CL_HI + WF_CL_NIN)
+#define WF_CL_ISTD (WF_CL_MEAN + WF_CL_NIN)
+#define WF_CL_W1 (WF_CL_ISTD + WF_CL_NIN)           /* [64][12] */
+#define WF_CL_B1 (WF_CL_W1 + WF_CL_NH * WF_CL_NIN)  /* [64] */
+#define WF_CL_W2 (WF_CL_B1 + WF_CL_NH)              /* [64][64] */
+#define WF_CL_B2 (WF_CL_W2 + WF_CL_NH * WF_CL_NH)   /* [64] */
+#define WF_CL_W3 (WF_CL_B2 + WF_CL_NH)              /* [64] */
+#define WF_CL_B3 (WF_CL_W3 + WF_CL_NH)
+#define WF_CL_SCALE (WF_CL_B3 + 1)
+#define WF_CL_NPARAMS (WF_CL_SCALE + 1)
+
+/* B-operand fragments, 16 bytes (8 bf16) per lane each:
+ *   W1: 4 output tiles x 1 K-step, W2: 4 x 2, w3: 1 x 2 (column 0 only) */
+#define WF_CL_FRAG_W1 0
+#define WF_CL_FRAG_W2 (WF_CL_FRAG_W1 + 4 * 64)
+#define WF_CL_FRAG_W3 (WF_CL_FRAG_W2 + 8 * 64)
+#define WF_CL_NFRAG (WF_CL_FRAG_W3 + 2 * 64)
+
+typedef __bf16 wf_bf16x8 __attribute__((ext_vector_type(8)));
+typedef float wf_f32x4 __attribute__((ext_vector_type(4)));
+
+/* Device view of a closure: the FP32 parameter block and the bf16 fragments. */
+struct WfClosure {
+    const float *prm;
+    const wf_bf16x8 *frag;
+};
+
+/* LDS per wavefront: inputs 64 x 32 bf16, hidden 16 x 64 bf16, outputs 64 floats */
+#define WF_CL_LDS_X (64 * 32)
+#define WF_CL_LDS_H (16 * 64)
+struct __attribute__((aligned(16))) WfClosureLds { /* fragments are read as 16-byte vectors */
+    __bf16 x[WF_CL_LDS_X];
+    __bf16 h[WF_CL_LDS_H];
+    float z[64];
+};
+
+/* with the hardware exponential and reciprocal (v_exp_f32, v_rcp_f32, about
+ * 1 ulp each): 128 of these per node would otherwise dominate the network's
+ * cost as full-precision division sequences */
+__device__ __forceinline__ float wf_cl_gelu(float h) {
+    return h * __builtin_amdgcn_rcpf(1.0f + __expf(-1.702f * h));
+}
+
+/* Exchange through this wavefront's own LDS block: DS instructions of one
+ * wavefront execute in order, so the only requirement is that the compiler
+ * keeps the accesses in program order and the writes are complete. */
+__device__ __forceinline__ void wf_cl_wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+__device__ __forceinline__ float wf_cl_softplus(float z) { return (z > 20.0f) ? z : log1pf(expf(z)); }
+
+/* Normalised input of one node (FP64 arithmetic, rounded to FP32 then bf16)
+ * and whether all raw inputs lie in the training range. */
+__device__ __forceinline__ bool wf_cl_inputs(const float *prm, const double f[WF_CL_NIN], __bf16 x[WF_CL_NIN]) {
+    bool in = true;
+#pragma unroll
+    for (int i = 0; i < WF_CL_NIN; i++) {
+        in = in && (f[i] >= (double)prm[WF_CL_LO + i]) && (f[i] <= (double)prm[WF_CL_HI + i]);
+        x[i] = (__bf16)(float)((f[i] - (double)prm[WF_CL_MEAN + i]) * (double)prm[WF_CL_ISTD + i]);
+    }
+    return in;
+}
+
+/* Network output z for the 64 nodes of this wavefront: lane l passes its
+ * node's normalised inputs and receives that node's z.  Every lane of the
+ * wavefront must call it (MFMA is wave-wide); L is this wavefront's LDS. */
+__device__ __forceinline__ float wf_cl_forward(const WfClosure &cl, WfClosureLds &L, const __bf16 x[WF_CL_NIN]) {
+    const int l = threadIdx.x & 63;
+    const int lr = l & 15, lg = l >> 4;
+    const float *prm = cl.prm;
+    /* this lane's row of the input matrix (padded to 32 columns), as four
+     * 16-byte stores */
+    {
+        wf_bf16x8 r0, r1, r2 = {}, r3 = {};
+#pragma unroll
+        for (int i = 0; i < 8; i++) r0[i] = x[i];
+#pragma unroll
+        for (int i = 0; i < 8; i++) r1[i] = (i < WF_CL_NIN - 8) ? x[8 + i] : (__bf16)0.0f;
+        wf_bf16x8 *row = (wf_bf16x8 *)&L.x[l * 32];
+        row[0] = r0;
+        row[1] = r1;
+        row[2] = r2;
+        row[3] = r3;
+    }
+    wf_cl_wave_sync();
+    const wf_bf16x8 *W1 = cl.frag + WF_CL_FRAG_W1, *W2 = cl.frag + WF_CL_FRAG_W2, *W3 = cl.frag + WF_CL_FRAG_W3;
+#pragma unroll 1
+    for (int mt = 0; mt < 4; mt++) { /* 16 nodes per tile */
+        const wf_bf16x8 a1 = *(const wf_bf16x8 *)&L.x[(mt * 16 + lr) * 32 + 8 * lg];
+        wf_f32x4 d[4];
+#pragma unroll
+        for (int nt = 0; nt < 4; nt++) {
+            d[nt] = wf_f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+            d[nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, W1[nt * 64 + l], d[nt], 0, 0, 0);
+        }
+        wf_cl_wave_sync(); /* the previous tile's reads of L.h are done */
+#pragma unroll
+        for (int nt = 0; nt < 4; nt++) {
+            const int n = nt * 16 + lr;
+            const float b = prm[WF_CL_B1 + n];
+#pragma unroll
+            for (int r = 0; r < 4; r++) L.h[(4 * lg + r) * 64 + n] = (__bf16)wf_cl_gelu(d[nt][r] + b);
+        }
+        wf_cl_wave_sync();
+        const wf_bf16x8 a2a = *(const wf_bf16x8 *)&L.h[lr * 64 + 8 * lg];
+        const wf_bf16x8 a2b = *(const wf_bf16x8 *)&L.h[lr * 64 + 32 + 8 * lg];
+#pragma unroll
+        for (int nt = 0; nt < 4; nt++) {
+            d[nt] = wf_f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+            d[nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a2a, W2[(nt * 2 + 0) * 64 + l], d[nt], 0, 0, 0);
+            d[nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a2b, W2[(nt * 2 + 1) * 64 + l], d[nt], 0, 0, 0);
+        }
+        wf_cl_wave_sync();
+#pragma unroll
+        for (int nt = 0; nt < 4; nt++) {
+            const int n = nt * 16 + lr;
+            const float b = prm[WF_CL_B2 + n];
+#pragma unroll
+            for (int r = 0; r < 4; r++) L.h[(4 * lg + r) * 64 + n] = (__bf16)wf_cl_gelu(d[nt][r] + b);
+        }
+        wf_cl_wave_sync();
+        const wf_bf16x8 a3a = *(const wf_bf16x8 *)&L.h[lr * 64 + 8 * lg];
+        const wf_bf16x8 a3b = *(const wf_bf16x8 *)&L.h[lr * 64 + 32 + 8 * lg];
+        wf_f32x4 z = wf_f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+        z = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a3a, W3[0 * 64 + l], z, 0, 0, 0);
+        z = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a3b, W3[1 * 64 + l], z, 0, 0, 0);
+        if (lr == 0) {
+#pragma unroll
+            for (int r = 0; r < 4; r++) L.z[mt * 16 + 4 * lg + r] = z[r] + prm[WF_CL_B3];
+        }
+    }
+    wf_cl_wave_sync();
+    return L.z[l];
+}
+
+/* nu_t from the network output, and the node's mu_sgs under the closure:
+ * -rho nu_t when the inputs are in range and nu_t is finite, else the
+ * Smagorinsky value mu_smag (>= 0). */
+__device__ __forceinline__ double wf_cl_mu(const float *prm, bool in_range, float z, double rho, double mu_smag) {
+    const float nu = prm[WF_CL_SCALE] * wf_cl_softplus(z);
+    const bool ok = in_range && (nu >= 0.0f) && (nu < 3.0e38f);
+    return ok ? -(rho * (double)nu) : mu_smag;
+}
+
+/* Viscosity the stage pass uses at a point: a learned value (stored
+ * negative) is taken as it is, a Smagorinsky value is damped by fw^2. */
+template <bool CL>
+__device__ __forceinline__ double wf_cl_damped(double mu, double fw) {
+    if constexpr (CL)
+        return (mu < 0.0) ? -mu : mu * (fw * fw);
+    else
+        return mu * (fw * fw);
+}
+
+#endif /* WF_CLOSURE_H */

@@ -38,6 +38,7 @@ struct WfConst {
     const double *zc;      /* WF_ZC z-stencil coefficients per plane (WF_Z_*) */
     const double *z;       /* Nz */
     const double *cd2;     /* Nz: (C_s (dx dy dz_k)^(1/3))^2 evaluated on the host */
+    const double *delta_k; /* Nz: filter width (dx dy dz_k)^(1/3), an input of the learned closure */
     const double *topo;    /* plane */
     const int *cut;        /* plane */
     const int *nzy;        /* plane */

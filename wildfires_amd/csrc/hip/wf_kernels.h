@@ -6,6 +6,7 @@
 #define WF_KERNELS_H
 
 #include "wf_common.h"
+#include "wf_closure.h"
 
 /* k_stage modes (RK combination written by the stage pass) */
 #define WF_MODE_PHI 0   /* out = k_s */
@@ -61,9 +62,15 @@ __device__ __forceinline__ double wf_heat_source(const WfConst &c, int col, int 
 }
 
 /* stage pass (wf_stage.hip) */
-hipError_t wf_launch_derived(const WfConst &c, const WfState &X, const WfDerived &D, hipStream_t s);
+hipError_t wf_launch_derived(const WfConst &c, const WfState &X, const WfDerived &D, const WfClosure *cl,
+                             hipStream_t s);
+/* library-internal helpers shared by the solver and the closure trainer */
+int wf_set_error(int code, const char *msg);
+uint16_t wf_bf16_bits(float f);
+hipError_t wf_launch_closure_eval(const WfClosure &cl, const double *feat, long n, double *nu, int *learned,
+                                  hipStream_t s);
 hipError_t wf_launch_stage(const WfConst &c, const WfState &X, const WfDerived &D, const WfState &Yn,
-                           const WfState &acc, const WfState &out, int mode, double a, hipStream_t s);
+                           const WfState &acc, const WfState &out, int mode, double a, bool closure, hipStream_t s);
 hipError_t wf_launch_to_internal(const double *src, double *dst, int Nx, int Ny, int nz, hipStream_t s);
 hipError_t wf_launch_to_reference(const double *src, double *dst, int Nx, int Ny, int nz, hipStream_t s);
 

@@ -11,6 +11,8 @@
  *             spectrum of the top pressure plane; U the Thomas pivots
  */
 #include <math.h>
+#include <stdint.h>
+#include <cmath>
 #include <stdarg.h>
 #include <stdio.h>
 #include <stdlib.h>
@@ -36,6 +38,9 @@ static int wf_fail(int code, const char *fmt, ...) {
     return code;
 }
 
+/* for the other translation units of the library (wf_closure_train.hip) */
+int wf_set_error(int code, const char *msg) { return wf_fail(code, "%s", msg); }
+
 #define WF_HIP(call)                                                                              \
     do {                                                                                          \
         hipError_t e_ = (call);                                                                   \
@@ -57,7 +62,11 @@ struct wf_solver {
     double *derived;
     double *p, *A, *UB, *pmax;
     double2 *F, *Ptop;
-    double *z, *cd2, *zc, *topo, *kx2, *ky2, *ta, *tc, *beta, *x, *y;
+    double *z, *cd2, *delta_k, *zc, *topo, *kx2, *ky2, *ta, *tc, *beta, *x, *y;
+    /* learned closure (off unless wf_solver_set_closure gave parameters) */
+    float *cl_prm;
+    wf_bf16x8 *cl_frag;
+    int closure_on;
     int *cut, *nzy, *radx, *rady;
     double2 *twx, *twy;
     double *scratch, *Tsrc, *rho;
@@ -233,10 +242,11 @@ static int wf_create_impl(wf_solver *s, const Parameters *P) {
 
     /* grid tables evaluated on the host exactly as the reference evaluates them */
     const double *z = P->z;
-    std::vector<double> cd2(Nz);
+    std::vector<double> cd2(Nz), delta_k(Nz);
     for (int k = 0; k < Nz; k++) {
         double dz_k = (k < Nz - 1) ? z[k + 1] - z[k] : z[k] - z[k - 1];
-        cd2[k] = pow(P->C_s * pow(P->dx * P->dy * dz_k, 1.0 / 3.0), 2.0);
+        delta_k[k] = pow(P->dx * P->dy * dz_k, 1.0 / 3.0);
+        cd2[k] = pow(P->C_s * delta_k[k], 2.0);
     }
     /* z-stencil coefficients: the weights of the reference's difference
      * formulas (src/cu/pde.cu:363-385, 497-500, 539-542, 572) per plane */
@@ -279,6 +289,7 @@ static int wf_create_impl(wf_solver *s, const Parameters *P) {
     }
     if ((rc = wf_upload(&s->z, z, Nz))) return rc;
     if ((rc = wf_upload(&s->cd2, cd2.data(), Nz))) return rc;
+    if ((rc = wf_upload(&s->delta_k, delta_k.data(), Nz))) return rc;
     if ((rc = wf_upload(&s->zc, zc.data(), zc.size()))) return rc;
     if ((rc = wf_upload(&s->topo, P->topography, s->plane))) return rc;
     if ((rc = wf_upload(&s->x, P->x, Nx))) return rc;
@@ -342,6 +353,7 @@ static int wf_create_impl(wf_solver *s, const Parameters *P) {
     c.exchange_on = (P->alpha_s * P->sigma_s != 0.0);
     c.z = s->z;
     c.cd2 = s->cd2;
+    c.delta_k = s->delta_k;
     c.zc = s->zc;
     c.inv2dx = 1.0 / (2 * P->dx);
     c.inv2dy = 1.0 / (2 * P->dy);
@@ -467,7 +479,8 @@ extern "C" int wf_solver_destroy(wf_solver *s) {
     if (s == NULL) return WF_OK;
     if (s->stream) (void)hipStreamSynchronize(s->stream);
     for (int q = 0; q < 4; q++) (void)hipFree(s->state[q]);
-    void *bufs[] = {s->zc, s->derived, s->p, s->A, s->UB, s->pmax, s->F, s->Ptop, s->z, s->cd2, s->topo, s->kx2, s->ky2, s->ta,
+    void *bufs[] = {s->zc, s->derived, s->p, s->A, s->UB, s->pmax, s->F, s->Ptop, s->z, s->cd2, s->delta_k,
+                    s->cl_prm, s->cl_frag, s->topo, s->kx2, s->ky2, s->ta,
                     s->tc, s->beta, s->x, s->y, s->cut, s->nzy, s->radx, s->rady, s->twx, s->twy, s->scratch,
                     s->Tsrc, s->rho, s->ctl};
     for (size_t q = 0; q < sizeof(bufs) / sizeof(bufs[0]); q++)
@@ -556,8 +569,9 @@ static int wf_stage(wf_solver *s, double *X, double *out, int mode, double a) {
     WfState xs = wf_view(s, X), yn = wf_view(s, s->state[s->i_yn]), acc = wf_view(s, s->state[s->i_acc]);
     WfState o = wf_view(s, out);
     WfDerived D = wf_derived_view(s);
-    WF_LAUNCH(s, WF_K_DERIVED, wf_launch_derived(s->c, xs, D, s->stream));
-    WF_LAUNCH(s, WF_K_STAGE, wf_launch_stage(s->c, xs, D, yn, acc, o, mode, a, s->stream));
+    const WfClosure cl{s->cl_prm, s->cl_frag};
+    WF_LAUNCH(s, WF_K_DERIVED, wf_launch_derived(s->c, xs, D, s->closure_on ? &cl : nullptr, s->stream));
+    WF_LAUNCH(s, WF_K_STAGE, wf_launch_stage(s->c, xs, D, yn, acc, o, mode, a, s->closure_on != 0, s->stream));
     return WF_OK;
 }
 
@@ -769,4 +783,100 @@ extern "C" int wf_solver_last_pressure(const wf_solver *s, double *error, int *i
     if (error) *error = s->last_err;
     if (iterations) *iterations = s->last_iters;
     return WF_OK;
+}
+
+/* ------------------------------------------------------- learned closure ---- */
+
+static_assert(WF_CL_NPARAMS == WF_CLOSURE_NPARAMS, "closure parameter block differs from the public layout");
+
+/* float -> bf16 bits, round to nearest even (NaN kept quiet) */
+uint16_t wf_bf16_bits(float f) {
+    uint32_t u;
+    memcpy(&u, &f, 4);
+    if ((u & 0x7f800000u) == 0x7f800000u && (u & 0x007fffffu)) return (uint16_t)((u >> 16) | 0x40);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+
+/* The MFMA B-operand fragments of the three layers (layout in wf_closure.h). */
+static void wf_closure_fragments(const float *prm, std::vector<uint16_t> &frag) {
+    frag.assign((size_t)WF_CL_NFRAG * 8, 0);
+    for (int l = 0; l < 64; l++) {
+        const int lr = l & 15, lg = l >> 4;
+        for (int j = 0; j < 8; j++) {
+            for (int nt = 0; nt < 4; nt++) {
+                const int n = nt * 16 + lr, k = 8 * lg + j;
+                const float w1 = (k < WF_CL_NIN) ? prm[WF_CL_W1 + n * WF_CL_NIN + k] : 0.0f;
+                frag[((size_t)(WF_CL_FRAG_W1 + nt * 64 + l)) * 8 + j] = wf_bf16_bits(w1);
+                for (int ks = 0; ks < 2; ks++) {
+                    const int k2 = ks * 32 + 8 * lg + j;
+                    frag[((size_t)(WF_CL_FRAG_W2 + (nt * 2 + ks) * 64 + l)) * 8 + j] =
+                        wf_bf16_bits(prm[WF_CL_W2 + n * WF_CL_NH + k2]);
+                }
+            }
+            for (int ks = 0; ks < 2; ks++) {
+                const int k3 = ks * 32 + 8 * lg + j;
+                frag[((size_t)(WF_CL_FRAG_W3 + ks * 64 + l)) * 8 + j] =
+                    wf_bf16_bits(lr == 0 ? prm[WF_CL_W3 + k3] : 0.0f);
+            }
+        }
+    }
+}
+
+static int wf_closure_upload(const float *params, long n, float **prm_dev, wf_bf16x8 **frag_dev) {
+    if (n != WF_CL_NPARAMS) return wf_fail(WF_E_ARG, "closure: %ld parameters given, %d expected", n, WF_CL_NPARAMS);
+    for (long q = 0; q < n; q++)
+        if (!std::isfinite(params[q])) return wf_fail(WF_E_ARG, "closure: parameter %ld is not finite", q);
+    std::vector<uint16_t> frag;
+    wf_closure_fragments(params, frag);
+    WF_HIP(hipMalloc((void **)prm_dev, sizeof(float) * n));
+    WF_HIP(hipMemcpy(*prm_dev, params, sizeof(float) * n, hipMemcpyHostToDevice));
+    WF_HIP(hipMalloc((void **)frag_dev, sizeof(wf_bf16x8) * WF_CL_NFRAG));
+    WF_HIP(hipMemcpy(*frag_dev, frag.data(), sizeof(wf_bf16x8) * WF_CL_NFRAG, hipMemcpyHostToDevice));
+    return WF_OK;
+}
+
+extern "C" int wf_solver_set_closure(wf_solver *s, const float *params, long n) {
+    if (s == NULL) return wf_fail(WF_E_ARG, "null solver");
+    WF_HIP(hipStreamSynchronize(s->stream));
+    if (s->cl_prm) WF_HIP(hipFree(s->cl_prm));
+    if (s->cl_frag) WF_HIP(hipFree(s->cl_frag));
+    s->cl_prm = NULL;
+    s->cl_frag = NULL;
+    s->closure_on = 0;
+    if (params == NULL) return WF_OK;
+    int rc = wf_closure_upload(params, n, &s->cl_prm, &s->cl_frag);
+    if (rc) return rc;
+    s->closure_on = 1;
+    return WF_OK;
+}
+
+extern "C" int wf_closure_eval(const float *params, long nparams, const double *features, long n, double *nu_t,
+                               int *learned) {
+    if (params == NULL || features == NULL || nu_t == NULL || learned == NULL || n < 0)
+        return wf_fail(WF_E_ARG, "null or negative argument");
+    if (n == 0) return WF_OK;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return wf_fail(WF_E_DEVICE, "no HIP device");
+    WF_HIP(hipSetDevice(0));
+    float *prm = NULL;
+    wf_bf16x8 *frag = NULL;
+    double *f = NULL, *nu = NULL;
+    int *lf = NULL;
+    int rc = wf_closure_upload(params, nparams, &prm, &frag);
+    hipError_t e = hipSuccess;
+    if (rc == WF_OK) {
+        e = hipMalloc((void **)&f, sizeof(double) * WF_CL_NIN * n);
+        if (e == hipSuccess) e = hipMalloc((void **)&nu, sizeof(double) * n);
+        if (e == hipSuccess) e = hipMalloc((void **)&lf, sizeof(int) * n);
+        if (e == hipSuccess) e = hipMemcpy(f, features, sizeof(double) * WF_CL_NIN * n, hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = wf_launch_closure_eval(WfClosure{prm, frag}, f, n, nu, lf, 0);
+        if (e == hipSuccess) e = hipMemcpy(nu_t, nu, sizeof(double) * n, hipMemcpyDeviceToHost);
+        if (e == hipSuccess) e = hipMemcpy(learned, lf, sizeof(int) * n, hipMemcpyDeviceToHost);
+        if (e != hipSuccess) rc = wf_fail(WF_E_DEVICE, "closure evaluation: %s", hipGetErrorString(e));
+    }
+    void *bufs[] = {prm, frag, f, nu, lf};
+    for (void *b : bufs)
+        if (b) (void)hipFree(b);
+    return rc;
 }

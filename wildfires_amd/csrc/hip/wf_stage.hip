@@ -20,6 +20,7 @@
  */
 #include "wf_common.h"
 #include "wf_kernels.h"
+#include "wf_closure.h"
 
 /* 32-bit element offsets from a uniform base (global loads with a scalar base
  * and a 32-bit lane offset; the solver limits Nx*Ny*Nz to 2^28). */
@@ -56,11 +57,20 @@ __device__ __forceinline__ double wf_heat_q(const WfConst &c, int k, int nzy, do
 
 /* ---------------------------------------------------------------- pass 1 ---- */
 
-__global__ __launch_bounds__(256) void k_derived(WfConst c, WfState X, WfDerived D) {
+template <bool CL>
+__global__ __launch_bounds__(256) void k_derived(WfConst c, WfState X, WfDerived D, WfClosure cl) {
 #pragma clang fp contract(fast)
     int i, j, k;
     wf_tile_ijk(c, &i, &j, &k);
-    if (i >= c.Nx || j >= c.Ny) return;
+    /* with the closure every lane of the tile takes part in the network
+     * (nodes past the grid evaluate a clamped node and store nothing) */
+    const bool inside = (i < c.Nx) && (j < c.Ny);
+    if constexpr (CL) {
+        i = (i < c.Nx) ? i : c.Nx - 1;
+        j = (j < c.Ny) ? j : c.Ny - 1;
+    } else {
+        if (!inside) return;
+    }
     const int Nx = c.Nx, Ny = c.Ny, Nz = c.Nz;
     const double *zk = c.zc + WF_ZC * k;
     const int im1 = wf_wrap_m(i, 1, Nx), ip1 = wf_wrap_p(i, 1, Nx);
@@ -130,7 +140,7 @@ __global__ __launch_bounds__(256) void k_derived(WfConst c, WfState X, WfDerived
     const double s23 = 0.5 * (wy + vz);
     const double mod_S = sqrt(2.0 * (ux * ux + vy * vy + wz * wz) + (uz + wx) * (uz + wx) + (vx + uy) * (vx + uy) +
                               (wy + vz) * (wy + vz));
-    const double mu_sgs = rho * c.cd2[k] * mod_S;
+    double mu_sgs = rho * c.cd2[k] * mod_S;
     /* T_RHS_tmp of src/cu/pde.cu:706 (needed for div_U) */
     double mu = c.mu, kT = c.kT, dk = 0.0;
     const double K_T = (T > c.T_pc) ? c.A * exp(-c.T_act * invT) : 0.0;
@@ -144,6 +154,16 @@ __global__ __launch_bounds__(256) void k_derived(WfConst c, WfState X, WfDerived
     const double T2 = T * T, T3 = T * T * T;
     const double T_RHS_tmp = (dk + 12 * WF_SIGMA_D * c.delta * T2 * (Tx * Tx + Ty * Ty + Tz * Tz) +
                               (kT + 4 * WF_SIGMA_D * c.delta * T3) * lap_T) * inv_rho * c.inv_cp + q_src;
+    if constexpr (CL) {
+        __shared__ WfClosureLds cl_lds[WF_TILE_Y];
+        const double f[WF_CL_NIN] = {ux, uy, uz, vx, vy, vz, wx, wy, wz,
+                                     sqrt(Tx * Tx + Ty * Ty + Tz * Tz), c.delta_k[k], Ynode};
+        __bf16 x[WF_CL_NIN];
+        const bool in_range = wf_cl_inputs(cl.prm, f, x);
+        const float z = wf_cl_forward(cl, cl_lds[threadIdx.y], x);
+        mu_sgs = wf_cl_mu(cl.prm, in_range, z, rho, mu_sgs);
+        if (!inside) return;
+    }
     D.mu[q] = mu_sgs;
     D.s11[q] = s11;
     D.s12[q] = s12;
@@ -325,7 +345,7 @@ struct WfGlobalAcc {
 /* Raw RHS of u, v, w, T, Y (RHS + SGS, before any boundary condition) at an
  * interior node 1 <= k <= Nz-2 (src/cu/pde.cu:139-787,
  * src/cu/turbulence.cu:255-454). */
-template <class Acc>
+template <bool CL, class Acc>
 __device__ __forceinline__ void wf_rhs_node(const WfConst &c, const Acc &a, int k, int nzy, double r[5]) {
 #pragma clang fp contract(fast)
     const int Nz = c.Nz;
@@ -406,10 +426,15 @@ __device__ __forceinline__ void wf_rhs_node(const WfConst &c, const Acc &a, int 
     const double fw_jm = wf_fw_inv(wf_zibm_t(c, a.tp(0, -1), k), a.ut(0, -1), inv_nu_t);
     const double fw_km = wf_fw_inv(wf_zibm_t(c, tp0, k - 1), ut0, inv_nu_t);
     const double fw_kp = wf_fw_inv(wf_zibm_t(c, tp0, k + 1), ut0, inv_nu_t);
-    const double mu_c = a.dz(WF_D_MU, 0) * (fw * fw);
-    const double mu_ip = a.dv(WF_D_MU, 1, 0) * (fw_ip * fw_ip), mu_im = a.dv(WF_D_MU, -1, 0) * (fw_im * fw_im);
-    const double mu_jp = a.dv(WF_D_MU, 0, 1) * (fw_jp * fw_jp), mu_jm = a.dv(WF_D_MU, 0, -1) * (fw_jm * fw_jm);
-    const double mu_kp = a.dz(WF_D_MU, 1) * (fw_kp * fw_kp), mu_km = a.dz(WF_D_MU, -1) * (fw_km * fw_km);
+    /* damped mu_sgs at the node and its neighbours (a learned value, stored
+     * negative under the closure, is not damped) */
+    const double mu_c = wf_cl_damped<CL>(a.dz(WF_D_MU, 0), fw);
+    const double mu_ip = wf_cl_damped<CL>(a.dv(WF_D_MU, 1, 0), fw_ip);
+    const double mu_im = wf_cl_damped<CL>(a.dv(WF_D_MU, -1, 0), fw_im);
+    const double mu_jp = wf_cl_damped<CL>(a.dv(WF_D_MU, 0, 1), fw_jp);
+    const double mu_jm = wf_cl_damped<CL>(a.dv(WF_D_MU, 0, -1), fw_jm);
+    const double mu_kp = wf_cl_damped<CL>(a.dz(WF_D_MU, 1), fw_kp);
+    const double mu_km = wf_cl_damped<CL>(a.dz(WF_D_MU, -1), fw_km);
     const double S11 = a.dz(WF_D_S11, 0), S12 = a.dz(WF_D_S12, 0), S13 = a.dz(WF_D_S13, 0);
     const double S22 = a.dz(WF_D_S22, 0), S23 = a.dz(WF_D_S23, 0), S33 = a.dz(WF_D_S33, 0);
     const double mu_x = (mu_ip - mu_im) * c.inv2dx;
@@ -560,6 +585,7 @@ __device__ __forceinline__ void wf_keep_raw(const WfConst &c, const WfDerived &D
 
 /* A node on the duplicate planes i = Nx-1 and j = Ny-1 (t indexes the
  * Nx + Ny - 1 columns of those planes), neighbours from global memory. */
+template <bool CL>
 __device__ __forceinline__ void wf_stage_dup_node(const WfConst &c, const WfState &X, const WfDerived &D,
                                                   const WfState &Yn, const WfState &acc, const WfState &out,
                                                   int mode, double a, int t, int k) {
@@ -580,7 +606,7 @@ __device__ __forceinline__ void wf_stage_dup_node(const WfConst &c, const WfStat
         r[4] = (k < nzy) ? c.Y_dead : 0.0;
     } else {
         const WfGlobalAcc acc_g{c, X, D, i, j, k};
-        wf_rhs_node(c, acc_g, k, nzy, r);
+        wf_rhs_node<CL>(c, acc_g, k, nzy, r);
         wf_keep_raw(c, D, col, k, cut, r);
     }
     wf_rk_store(mode, a, (unsigned)q, r, (k < c.Nzm) ? 5 : 4, rk, acc, out);
@@ -598,6 +624,7 @@ __device__ __forceinline__ void wf_stage_dup_node(const WfConst &c, const WfStat
 #ifndef WF_STAGE_MERGE_DUP
 #define WF_STAGE_MERGE_DUP 0
 #endif
+template <bool CL>
 __global__ __launch_bounds__(WF_NT_STAGE) WF_STAGE_ATTR void k_stage(WfConst c, WfState X, WfDerived D,
                                                                      WfState Yn, WfState acc, WfState out,
                                                                      int mode, double a, int ndup) {
@@ -606,7 +633,7 @@ __global__ __launch_bounds__(WF_NT_STAGE) WF_STAGE_ATTR void k_stage(WfConst c, 
     if ((int)blockIdx.x < ndup) {
         const int nd = c.Nx + c.Ny - 1;
         const int d = (int)blockIdx.x * WF_NT_STAGE + tid;
-        if (d < nd * (c.Nz - 1)) wf_stage_dup_node(c, X, D, Yn, acc, out, mode, a, d % nd, d / nd);
+        if (d < nd * (c.Nz - 1)) wf_stage_dup_node<CL>(c, X, D, Yn, acc, out, mode, a, d % nd, d / nd);
         return;
     }
 #endif
@@ -697,7 +724,7 @@ __global__ __launch_bounds__(WF_NT_STAGE) WF_STAGE_ATTR void k_stage(WfConst c, 
         r[4] = (k < nzy) ? c.Y_dead : 0.0;
     } else {
         const WfTileAcc acc_t{L, V, (ty + WF_SH) * WF_SW + (tx + WF_SH), (ty + WF_DH) * WF_DW + (tx + WF_DH)};
-        wf_rhs_node(c, acc_t, k, nzy, r);
+        wf_rhs_node<CL>(c, acc_t, k, nzy, r);
         wf_keep_raw(c, D, col, k, cut, r);
     }
     wf_rk_store(mode, a, q, r, (k < c.Nzm) ? 5 : 4, rk, acc, out);
@@ -705,11 +732,12 @@ __global__ __launch_bounds__(WF_NT_STAGE) WF_STAGE_ATTR void k_stage(WfConst c, 
 
 /* The duplicate-plane nodes of k_stage in a launch of their own: one thread
  * per node, (Nx + Ny - 1) columns x (Nz - 1) planes. */
+template <bool CL>
 __global__ __launch_bounds__(256) void k_stage_dup(WfConst c, WfState X, WfDerived D, WfState Yn, WfState acc,
                                                    WfState out, int mode, double a) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= c.Nx + c.Ny - 1) return;
-    wf_stage_dup_node(c, X, D, Yn, acc, out, mode, a, t, blockIdx.y);
+    wf_stage_dup_node<CL>(c, X, D, Yn, acc, out, mode, a, t, blockIdx.y);
 }
 
 /* Extrapolation giving a zero one-sided z-derivative at the bottom of the fluid
@@ -778,33 +806,66 @@ __global__ void k_to_reference(const double *src, double *dst, int Nx, int Ny, i
 
 /* ------------------------------------------------------------- launchers ---- */
 
-hipError_t wf_launch_derived(const WfConst &c, const WfState &X, const WfDerived &D, hipStream_t s) {
+hipError_t wf_launch_derived(const WfConst &c, const WfState &X, const WfDerived &D, const WfClosure *cl,
+                             hipStream_t s) {
     dim3 blk(WF_TILE_X, WF_TILE_Y, 1), grd(wf_tile_count(c));
-    hipLaunchKernelGGL(k_derived, grd, blk, 0, s, c, X, D);
+    if (cl)
+        hipLaunchKernelGGL(k_derived<true>, grd, blk, 0, s, c, X, D, *cl);
+    else
+        hipLaunchKernelGGL(k_derived<false>, grd, blk, 0, s, c, X, D, WfClosure{nullptr, nullptr});
+    return hipGetLastError();
+}
+
+template <bool CL>
+static hipError_t wf_launch_stage_t(const WfConst &c, const WfState &X, const WfDerived &D, const WfState &Yn,
+                                    const WfState &acc, const WfState &out, int mode, double a, hipStream_t s) {
+    const int tiles = ((c.Nx + WF_TILE_X - 1) / WF_TILE_X) * ((c.Ny + WF_TILE_Y - 1) / WF_TILE_Y) * (c.Nz - 1);
+    int ndup = 0;
+    if (WF_STAGE_MERGE_DUP) {
+        ndup = ((c.Nx + c.Ny - 1) * (c.Nz - 1) + WF_NT_STAGE - 1) / WF_NT_STAGE;
+    } else {
+        hipLaunchKernelGGL(k_stage_dup<CL>, dim3((c.Nx + c.Ny - 1 + 255) / 256, c.Nz - 1), dim3(256), 0, s, c, X, D,
+                           Yn, acc, out, mode, a);
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL(k_stage<CL>, dim3(ndup + tiles), dim3(WF_TILE_X, WF_TILE_Y, 1), 0, s, c, X, D, Yn, acc, out,
+                       mode, a, ndup);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_stage_columns, dim3((c.Nx + 63) / 64, c.Ny), dim3(64), 0, s, c, X, D, Yn, acc, out, mode, a);
     return hipGetLastError();
 }
 
 hipError_t wf_launch_stage(const WfConst &c, const WfState &X, const WfDerived &D, const WfState &Yn,
-                           const WfState &acc, const WfState &out, int mode, double a, hipStream_t s) {
-    const int tiles = ((c.Nx + WF_TILE_X - 1) / WF_TILE_X) * ((c.Ny + WF_TILE_Y - 1) / WF_TILE_Y) * (c.Nz - 1);
-#if WF_STAGE_MERGE_DUP
-    const int ndup = ((c.Nx + c.Ny - 1) * (c.Nz - 1) + WF_NT_STAGE - 1) / WF_NT_STAGE;
-#else
-    const int ndup = 0;
-    hipLaunchKernelGGL(k_stage_dup, dim3((c.Nx + c.Ny - 1 + 255) / 256, c.Nz - 1), dim3(256), 0, s, c, X, D, Yn, acc,
-                       out, mode, a);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
-#endif
-    hipLaunchKernelGGL(k_stage, dim3(ndup + tiles), dim3(WF_TILE_X, WF_TILE_Y, 1), 0, s, c, X, D, Yn, acc, out, mode,
-                       a, ndup);
-#if WF_STAGE_MERGE_DUP
-    hipError_t e = hipGetLastError();
-#else
-    e = hipGetLastError();
-#endif
-    if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_stage_columns, dim3((c.Nx + 63) / 64, c.Ny), dim3(64), 0, s, c, X, D, Yn, acc, out, mode, a);
+                           const WfState &acc, const WfState &out, int mode, double a, bool closure, hipStream_t s) {
+    return closure ? wf_launch_stage_t<true>(c, X, D, Yn, acc, out, mode, a, s)
+                   : wf_launch_stage_t<false>(c, X, D, Yn, acc, out, mode, a, s);
+}
+
+/* The network alone on n rows of raw inputs (row-major n x 12): nu_t per row
+ * and whether the row is in the training range with a finite output. */
+__global__ __launch_bounds__(256) void k_closure_eval(WfClosure cl, const double *feat, long n, double *nu,
+                                                      int *learned) {
+    __shared__ WfClosureLds lds[4];
+    const long row = (long)blockIdx.x * 256 + threadIdx.x;
+    const long rc = (row < n) ? row : n - 1;
+    double f[WF_CL_NIN];
+#pragma unroll
+    for (int i = 0; i < WF_CL_NIN; i++) f[i] = feat[rc * WF_CL_NIN + i];
+    __bf16 x[WF_CL_NIN];
+    const bool in_range = wf_cl_inputs(cl.prm, f, x);
+    const float z = wf_cl_forward(cl, lds[threadIdx.x >> 6], x);
+    if (row >= n) return;
+    const float v = cl.prm[WF_CL_SCALE] * wf_cl_softplus(z);
+    nu[row] = (double)v;
+    learned[row] = (in_range && v >= 0.0f && v < 3.0e38f) ? 1 : 0;
+}
+
+hipError_t wf_launch_closure_eval(const WfClosure &cl, const double *feat, long n, double *nu, int *learned,
+                                  hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_closure_eval, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, cl, feat, n, nu, learned);
     return hipGetLastError();
 }
 
