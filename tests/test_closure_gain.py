@@ -1,0 +1,70 @@
+"""What the trained closure gains over Smagorinsky, on the CPU.
+
+The network is the committed wildfires_amd/data/closure.npz, trained on the
+GPU by `python -m wildfires_amd.train_closure` (which also wrote the fixture
+tests/golden/closure_holdout.npz: 4000 rows of the held-out runs, inputs and
+the box-filtered fine-run target nu).  The oracle evaluates the network here;
+the GPU evaluation of the same rows is compared with it in
+test_gpu_closure.py.
+
+The two methods differ only on rows inside the training range (elsewhere the
+closure is Smagorinsky), so the scores are taken over those rows."""
+import os
+
+import numpy as np
+import pytest
+
+from conftest import ROOT
+from wildfires_amd import closure
+from wildfires_amd.train_closure import smagorinsky
+
+TRAINED = os.path.join(ROOT, "wildfires_amd", "data", "closure.npz")
+HOLDOUT = os.path.join(ROOT, "tests", "golden", "closure_holdout.npz")
+
+
+@pytest.fixture(scope="module")
+def held_out():
+    if not (os.path.exists(TRAINED) and os.path.exists(HOLDOUT)):
+        pytest.fail("trained closure or held-out fixture missing")
+    from oracle import oracle
+    block = closure.load(TRAINED)
+    with np.load(HOLDOUT, allow_pickle=False) as f:
+        F = f["features"].astype(np.float64)
+        nu = f["nu"].astype(np.float64)
+        C_s = float(f["C_s"])
+    nu_l, learned = oracle.closure_eval(block, F)
+    return F, nu, nu_l, learned == 1, smagorinsky(F, C_s)
+
+
+def test_network_covers_held_out_rows(held_out):
+    _, _, _, L, _ = held_out
+    assert L.mean() >= 0.95
+
+
+def test_closure_beats_smagorinsky_on_held_out_rows(held_out):
+    _, nu, nu_l, L, nu_s = held_out
+    rmse_l = np.sqrt(np.mean((nu_l[L] - nu[L]) ** 2))
+    rmse_s = np.sqrt(np.mean((nu_s[L] - nu[L]) ** 2))
+    corr_l = np.corrcoef(nu_l[L], nu[L])[0, 1]
+    corr_s = np.corrcoef(nu_s[L], nu[L])[0, 1]
+    # Smagorinsky turns the resolved wind shear into eddy viscosity that the
+    # filtered fine run does not carry: its error is orders of magnitude larger
+    assert rmse_l < 0.01 * rmse_s
+    assert corr_l > corr_s + 0.3
+    assert corr_l > 0.6
+
+
+def test_closure_tracks_active_rows(held_out):
+    _, nu, nu_l, L, nu_s = held_out
+    a = L & (nu > 1e-6)
+    assert a.sum() > 200
+    corr_l = np.corrcoef(nu_l[a], nu[a])[0, 1]
+    corr_s = np.corrcoef(nu_s[a], nu[a])[0, 1]
+    assert corr_l > corr_s
+    # the mean over the active rows (where the dissipation is): the network
+    # is within a factor 2 (it runs low, as the targets above the 99.5th
+    # percentile were clipped in training), Smagorinsky is over 10x high
+    r_l = np.mean(nu_l[a]) / np.mean(nu[a])
+    r_s = np.mean(nu_s[a]) / np.mean(nu[a])
+    assert 0.5 < r_l < 2.0
+    assert r_s > 10.0

@@ -143,3 +143,64 @@ def test_trainer_learns_a_known_viscosity():
     print("loss", loss0, "->", loss1, "median rel err", np.median(rel))
     assert loss1 <= 0.1 * loss0
     assert np.median(rel) <= 0.2
+
+
+def test_trained_closure_on_held_out_rows_matches_oracle():
+    """The committed trained network on the held-out fixture rows: GPU
+    against the oracle, and the gain over Smagorinsky from the GPU values."""
+    from oracle import oracle
+    from test_closure_gain import HOLDOUT, TRAINED
+    from wildfires_amd import api
+    from wildfires_amd.train_closure import smagorinsky
+    block = closure.load(TRAINED)
+    with np.load(HOLDOUT, allow_pickle=False) as f:
+        F = f["features"].astype(np.float64)
+        nu = f["nu"].astype(np.float64)
+        C_s = float(f["C_s"])
+    print("trained closure: %d held-out rows" % len(nu), flush=True)
+    nu_g, l_g = api.closure_eval(block, F)
+    nu_o, l_o = oracle.closure_eval(block, F)
+    assert np.array_equal(l_g, l_o)
+    L = l_g == 1
+    rel = np.abs(nu_g[L] - nu_o[L]) / np.maximum(np.abs(nu_o[L]), 1e-30)
+    print("median/max rel", np.median(rel), rel.max())
+    assert np.median(rel) <= 1e-5 and rel.max() <= 1e-2
+    nu_s = smagorinsky(F, C_s)
+    assert np.corrcoef(nu_g[L], nu[L])[0, 1] > np.corrcoef(nu_s[L], nu[L])[0, 1] + 0.3
+    assert np.sqrt(np.mean((nu_g[L] - nu[L]) ** 2)) < 0.01 * np.sqrt(np.mean((nu_s[L] - nu[L]) ** 2))
+
+
+def test_trained_closure_a_posteriori():
+    """A held-out scenario (one the network did not see) on the coarse grid
+    with Smagorinsky and with the trained closure, against the fine (2x) run
+    box-filtered to the coarse grid: at 4 s the closure's resolved kinetic
+    energy and its heated-front speed (2 s to 4 s) are the closer ones."""
+    import json
+    import os
+    from test_closure_gain import TRAINED
+    from wildfires_amd import train_closure as tc
+    block = closure.load(TRAINED)
+    with open(os.path.join(os.path.dirname(TRAINED), "closure_report.json")) as f:
+        rep = json.load(f)
+    r = len(rep["runs"]) - rep["held_out"]  # the first held-out run
+    u_r, x_ign = rep["runs"][r]["u_r"], rep["runs"][r]["x_ignition"]
+    print("held-out scenario u_r %.3f m/s, line at x %.2f m: fine run" % (u_r, x_ign), flush=True)
+    from conftest import short_dir
+    wd = short_dir()  # the output path must fit the 64-byte field of Parameters
+    xc = np.linspace(0.0, tc.COARSE["x_max"], tc.COARSE["Nx"])[:-1]
+    prm, snaps = tc.fine_run(wd, u_r, x_ign, "f")
+    fine = {t: tc.filtered_diagnostics(prm, y, xc) for t, y in snaps}
+    print("coarse runs", flush=True)
+    smag = tc.coarse_run(wd, u_r, x_ign, "s", None)
+    clos = tc.coarse_run(wd, u_r, x_ign, "c", block)
+    t_a, t_b = tc.SAMPLE_TIMES[1], tc.SAMPLE_TIMES[-1]
+
+    def ros(d):
+        return (d[t_b]["front_x"] - d[t_a]["front_x"]) / (t_b - t_a)
+
+    for name, d in (("fine", fine), ("smagorinsky", smag), ("closure", clos)):
+        print(name, "KE %.4f J/kg, front speed %.3f m/s" % (d[t_b]["kinetic_energy"], ros(d)))
+    assert all(d[t_b] is not None for d in (smag, clos))
+    ke = fine[t_b]["kinetic_energy"]
+    assert abs(clos[t_b]["kinetic_energy"] - ke) < abs(smag[t_b]["kinetic_energy"] - ke)
+    assert abs(ros(clos) - ros(fine)) < abs(ros(smag) - ros(fine))
