@@ -106,6 +106,7 @@ def run(args):
     log("timed %d steps in %.3f s: %.1f Mcell-steps/s, %.1f pressure solves/step" % (args.steps, elapsed, rate, solves))
 
     # per-kernel timing (HIP events on the solver stream) over profile_steps more steps
+    y_ref = solver.get_state() if not args.no_mixed else None  # FP64 state after warmup + steps
     solver.profile(True)
     solver.step(n, max(args.profile_steps, 1))
     times = solver.kernel_times()
@@ -155,9 +156,73 @@ def run(args):
     if not args.no_closure:
         result["learned_closure"] = closure_leg(solver, params, y0, p0, N, args)
     solver.close()
+    if not args.no_mixed:
+        result["mixed"] = mixed_leg(params, y0, p0, N, args, y_ref, rate, None)
+        if not args.no_closure:
+            result["mixed_learned_closure"] = mixed_leg(params, y0, p0, N, args, y_ref, rate, closure_block()[0])
     if not args.no_cpu_baseline:
         result["cpu_baseline"] = cpu_baseline(params, y0, p0)
     return result
+
+
+def closure_block():
+    """The committed trained closure, or a seeded untrained one."""
+    from wildfires_amd import closure
+    path = os.path.join(ROOT, "wildfires_amd", "data", "closure.npz")
+    if os.path.exists(path):
+        return closure.load(path), "trained (wildfires_amd/data/closure.npz)"
+    return closure.random_block(0, out_scale=1e-3), "untrained (seeded random, full input range)"
+
+
+def mixed_leg(params, y0, p0, N, args, y_ref, rate64, block):
+    """The mixed form (FP32 fields, stencils, FFTs and fuel update; FP64 Y,
+    lowest-wavenumber tridiagonal solves and reductions) on the same workload
+    and step count, optionally with the learned closure: Mcell-steps/s, the
+    ratio to the FP64 rate, per-kernel mean times and the relative L2 error of
+    each field against the FP64 run after warmup + steps steps."""
+    import numpy as np
+    from wildfires_amd import api
+    S = api.Solver(params, precision="mixed")
+    S.set_state(y0, p0)
+    if block is not None:
+        S.set_closure(block)
+    n = 1
+    if args.warmup:
+        S.step(n, args.warmup)
+        n += args.warmup
+    s0 = S.stats()
+    api.synchronize()
+    t0 = time.perf_counter()
+    S.step(n, args.steps)
+    api.synchronize()
+    elapsed = time.perf_counter() - t0
+    n += args.steps
+    s1 = S.stats()
+    y, p = S.get_state()
+    S.profile(True)
+    S.step(n, 1)
+    times = S.kernel_times()
+    S.profile(False)
+    S.close()
+    rate = N * args.steps / elapsed / 1e6
+    out = {"value": round(rate, 3), "unit": "Mcell-steps/s", "ms_per_step": round(1000.0 * elapsed / args.steps, 4),
+           "vs_fp64": round(rate / rate64, 3),
+           "dtype": "f32 u,v,w,T,p, stencils, FFT, fuel update; f64 Y, lowest-wavenumber Thomas, reductions"
+                    + ("; bf16 x bf16 -> f32 MFMA closure network" if block is not None else ""),
+           "pressure_solves_per_step": round((s1["pressure_solves"] - s0["pressure_solves"]) / args.steps, 3),
+           "kernels_mean_us": {k: round(1000.0 * ms / c, 2) for k, (ms, c) in times.items() if c},
+           "state_finite": bool(np.isfinite(y).all())}
+    if y_ref is not None:
+        def rel(a, b):
+            d = np.linalg.norm(b)
+            return float(np.linalg.norm(a - b) / d) if d > 0 else float(np.linalg.norm(a - b))
+        errs = {nm: rel(a, b) for nm, a, b in zip("uvwTY", params.split(y), params.split(y_ref[0]))}
+        errs["p"] = rel(p, y_ref[1])
+        out["rel_l2_vs_fp64_after_%d_steps" % (args.warmup + args.steps)] = {k: float("%.3e" % v)
+                                                                            for k, v in errs.items()}
+    log("mixed%s leg: %d steps in %.3f s: %.1f Mcell-steps/s" % (" + closure" if block is not None else "",
+                                                                 args.steps, elapsed, rate))
+    return out
 
 
 def closure_leg(solver, params, y0, p0, N, args):
@@ -165,12 +230,7 @@ def closure_leg(solver, params, y0, p0, N, args):
     (BF16 MFMA network in the derived pass, FP64 elsewhere): Mcell-steps/s,
     and the derived pass's mean launch time next to the plain one."""
     import numpy as np
-    from wildfires_amd import closure
-    path = os.path.join(ROOT, "wildfires_amd", "data", "closure.npz")
-    if os.path.exists(path):
-        block, weights = closure.load(path), "trained (wildfires_amd/data/closure.npz)"
-    else:
-        block, weights = closure.random_block(0, out_scale=1e-3), "untrained (seeded random, full input range)"
+    block, weights = closure_block()
     solver.set_state(y0, p0)
     solver.set_closure(block)
     n = 1
@@ -387,7 +447,8 @@ def main():
     ap.add_argument("--profile-steps", type=int, default=2)
     ap.add_argument("--workload", default="hill256", choices=["hill256", "flat64"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--no-closure", action="store_true", help="skip the learned-closure leg")
+    ap.add_argument("--no-closure", action="store_true", help="skip the learned-closure legs")
+    ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision legs")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run device-to-device copies of BYTES (counter scale factor under rocprofv3 --pmc)")
     ap.add_argument("--rocprof-summary", metavar="DIR",

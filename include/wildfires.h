@@ -304,6 +304,17 @@ long wf_state_size(const Parameters *parameters);
 int wf_solver_create(const Parameters *parameters, wf_solver **out);
 int wf_solver_destroy(wf_solver *solver);
 
+/* Precision of a solver.  WF_PRECISION_FP64 is the form that matches the
+ * reference (what wf_solver_create and solve_PDE use).  WF_PRECISION_MIXED has
+ * no reference counterpart: u, v, w, T, p, the stencils, FFTs and fuel update in
+ * FP32; the fuel fraction Y stored and accumulated in FP64, and the tridiagonal
+ * solves of the lowest wavenumbers and the convergence reductions in FP64.  The
+ * host interface (FP64 arrays in the reference layout) is the same for both. */
+#define WF_PRECISION_FP64 0
+#define WF_PRECISION_MIXED 1
+int wf_solver_create_ex(const Parameters *parameters, int precision, wf_solver **out);
+int wf_solver_precision(const wf_solver *solver);
+
 /* Host <-> device state transfer (reference layout on the host side). */
 int wf_solver_set_state(wf_solver *solver, const double *y_n, const double *p);
 int wf_solver_get_state(wf_solver *solver, double *y_n, double *p);

@@ -234,3 +234,15 @@ def test_solve_pde_fails_loudly_without_device(tmp_path):
     r = _driver(tmp_path, BASE)
     assert r.returncode == 1
     assert "Device setup failed" in r.stdout
+
+
+def test_solver_precision_is_checked_before_the_device(tmp_path):
+    """An unknown precision is an argument error (no device needed); the
+    Python binding rejects it before the C call."""
+    lib = _abi.load()
+    prm = make_params(tmp_path, "small_flat")
+    h = ctypes.c_void_p()
+    assert lib.wf_solver_create_ex(prm.ptr, 7, ctypes.byref(h)) == -1
+    assert b"precision" in lib.wf_last_error()
+    with pytest.raises(ValueError):
+        api.Solver(prm, precision="fp16")

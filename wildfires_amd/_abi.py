@@ -106,6 +106,8 @@ EXPORTS = [
     ("wf_parameters_sizeof", ctypes.c_size_t, []),
     ("wf_state_size", ctypes.c_long, [PARAMS_P]),
     ("wf_solver_create", ctypes.c_int, [PARAMS_P, ctypes.POINTER(ctypes.c_void_p)]),
+    ("wf_solver_create_ex", ctypes.c_int, [PARAMS_P, ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)]),
+    ("wf_solver_precision", ctypes.c_int, [ctypes.c_void_p]),
     ("wf_solver_destroy", ctypes.c_int, [ctypes.c_void_p]),
     ("wf_solver_set_state", ctypes.c_int, [ctypes.c_void_p, DBL_P, DBL_P]),
     ("wf_solver_get_state", ctypes.c_int, [ctypes.c_void_p, DBL_P, DBL_P]),

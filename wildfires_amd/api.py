@@ -170,13 +170,22 @@ def pack_state(params, u, v, w, T, Y):
     return y0
 
 
-class Solver:
-    """Device solver for one parameter set (wf_solver_*)."""
+PRECISIONS = {"fp64": 0, "mixed": 1}  # WF_PRECISION_FP64, WF_PRECISION_MIXED
 
-    def __init__(self, params):
+
+class Solver:
+    """Device solver for one parameter set (wf_solver_*).  precision "fp64" is
+    the form that matches the reference; "mixed" runs u, v, w, T, p, the
+    stencils, FFTs and fuel update in FP32, with Y, the lowest-wavenumber
+    tridiagonal solves and the convergence reductions in FP64."""
+
+    def __init__(self, params, precision="fp64"):
+        if precision not in PRECISIONS:
+            raise ValueError("precision must be one of %s" % sorted(PRECISIONS))
         self.params = params
+        self.precision = precision
         self.handle = ctypes.c_void_p()
-        _check(lib().wf_solver_create(params.ptr, ctypes.byref(self.handle)))
+        _check(lib().wf_solver_create_ex(params.ptr, PRECISIONS[precision], ctypes.byref(self.handle)))
 
     def close(self):
         if self.handle:

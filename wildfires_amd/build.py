@@ -16,7 +16,12 @@ LIB = os.path.join(PKG, "libwildfires_amd.so")
 EXE = os.path.join(PKG, "wildfires_amd")
 
 HOST_SOURCES = ["params.c", "terrain.c", "init.c", "output.c", "solver.c"]
-HIP_SOURCES = ["wf_stage.hip", "wf_pressure.hip", "wf_project.hip", "wf_solver.hip", "wf_closure_train.hip"]
+# compiled once per precision: WF_MIXED=0 (namespace wf64, FP64) and WF_MIXED=1
+# (namespace wf32, the mixed form; any implicit FP32 -> FP64 promotion is an error)
+HIP_PRECISION_SOURCES = ["wf_stage.hip", "wf_pressure.hip", "wf_project.hip", "wf_solver.hip"]
+# compiled once: the C-ABI over both precisions and the closure trainer
+HIP_SOURCES = ["wf_api.hip", "wf_closure_train.hip"]
+MIXED_FLAGS = ["-DWF_MIXED=1", "-Wdouble-promotion", "-Werror=double-promotion"]
 
 # FP64 kernels are compiled without FMA contraction (expressions evaluate in
 # the written order) and without packed FP32 instructions.
@@ -49,6 +54,11 @@ def build(verbose=False, jobs=None):
         obj = os.path.join(OUT, src + ".o")
         cmds.append([_hipcc()] + HIPCC_FLAGS + ["-c", os.path.join(CSRC, "hip", src), "-o", obj])
         objs.append(obj)
+    for src in HIP_PRECISION_SOURCES:
+        for tag, flags in (("64", ["-DWF_MIXED=0"]), ("32", MIXED_FLAGS)):
+            obj = os.path.join(OUT, src + "." + tag + ".o")
+            cmds.append([_hipcc()] + HIPCC_FLAGS + flags + ["-c", os.path.join(CSRC, "hip", src), "-o", obj])
+            objs.append(obj)
     with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
         list(ex.map(lambda c: _run(c, verbose), cmds))
     _run([_hipcc(), "--offload-arch=gfx950", "-shared", "-fPIC", "-o", LIB + ".tmp"] + objs + ["-lm"], verbose)

@@ -1,0 +1,204 @@
+/*
+ * wf_api.hip -- the wf_* C-ABI: error state, the solver handle over the two
+ * compilations of the solver core (wf64: FP64; wf32: the mixed form), and the
+ * learned closure's upload and stand-alone evaluation.
+ */
+#include <math.h>
+#include <stdarg.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <cmath>
+#include <vector>
+
+#include "../../../include/wildfires.h"
+#include "wf_core.h"
+#include "wf_kernels.h"
+
+#define WF_VERSION "wildfires_amd 0.2 (gfx950; FP64 and mixed FP32/FP64)"
+
+static thread_local char wf_err[512] = "";
+
+int wf_fail(int code, const char *fmt, ...) {
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(wf_err, sizeof(wf_err), fmt, ap);
+    va_end(ap);
+    return code;
+}
+const char *wf_error_message(void) { return wf_err; }
+void wf_error_restore(const char *msg) { snprintf(wf_err, sizeof(wf_err), "%s", msg); }
+int wf_set_error(int code, const char *msg) { return wf_fail(code, "%s", msg); }
+
+#define WF_HIP(call)                                                                              \
+    do {                                                                                          \
+        hipError_t e_ = (call);                                                                   \
+        if (e_ != hipSuccess)                                                                     \
+            return wf_fail(WF_E_DEVICE, "%s:%d %s: %s", __FILE__, __LINE__, #call, hipGetErrorString(e_)); \
+    } while (0)
+
+extern "C" const char *wf_last_error(void) { return wf_err; }
+extern "C" const char *wf_version(void) { return WF_VERSION; }
+
+/* ------------------------------------------------------------ solver handle ---- */
+
+struct wf_solver {
+    int precision;
+    wf64::Solver *s64;
+    wf32::Solver *s32;
+};
+
+/* run CALL(core, ...) on the core of the handle's precision */
+#define WF_DISPATCH(h, CALL, ...)                                                              \
+    do {                                                                                       \
+        if ((h) == NULL) return wf_fail(WF_E_ARG, "null solver");                              \
+        return ((h)->precision == WF_PRECISION_MIXED) ? wf32::CALL((h)->s32, __VA_ARGS__)      \
+                                                      : wf64::CALL((h)->s64, __VA_ARGS__);     \
+    } while (0)
+
+extern "C" int wf_solver_create_ex(const Parameters *parameters, int precision, wf_solver **out) {
+    if (parameters == NULL || out == NULL) return wf_fail(WF_E_ARG, "null argument");
+    *out = NULL;
+    if (precision != WF_PRECISION_FP64 && precision != WF_PRECISION_MIXED)
+        return wf_fail(WF_E_ARG, "unknown precision %d", precision);
+    wf_solver *h = (wf_solver *)calloc(1, sizeof(wf_solver));
+    if (h == NULL) return wf_fail(WF_E_ALLOC, "out of host memory");
+    h->precision = precision;
+    const int rc = (precision == WF_PRECISION_MIXED) ? wf32::create(parameters, &h->s32)
+                                                     : wf64::create(parameters, &h->s64);
+    if (rc != WF_OK) {
+        free(h);
+        return rc;
+    }
+    *out = h;
+    return WF_OK;
+}
+
+extern "C" int wf_solver_create(const Parameters *parameters, wf_solver **out) {
+    return wf_solver_create_ex(parameters, WF_PRECISION_FP64, out);
+}
+
+extern "C" int wf_solver_precision(const wf_solver *h) {
+    if (h == NULL) return wf_fail(WF_E_ARG, "null solver");
+    return h->precision;
+}
+
+extern "C" int wf_solver_destroy(wf_solver *h) {
+    if (h == NULL) return WF_OK;
+    if (h->s64) wf64::destroy(h->s64);
+    if (h->s32) wf32::destroy(h->s32);
+    free(h);
+    return WF_OK;
+}
+
+extern "C" int wf_solver_set_state(wf_solver *h, const double *y_n, const double *p) { WF_DISPATCH(h, set_state, y_n, p); }
+extern "C" int wf_solver_get_state(wf_solver *h, double *y_n, double *p) { WF_DISPATCH(h, get_state, y_n, p); }
+extern "C" int wf_solver_step(wf_solver *h, int step0, int n_steps) { WF_DISPATCH(h, step, step0, n_steps); }
+extern "C" int wf_solver_phi(wf_solver *h, const double *R_old, double *R_new) { WF_DISPATCH(h, phi, R_old, R_new); }
+extern "C" int wf_solver_pressure(wf_solver *h, const double *y_np1, const double *y_n, double *p, double *error,
+                                  int *iterations) {
+    WF_DISPATCH(h, pressure, y_np1, y_n, p, error, iterations);
+}
+extern "C" int wf_solver_project(wf_solver *h, double *y_np1, const double *y_n, const double *p, int n) {
+    WF_DISPATCH(h, project, y_np1, y_n, p, n);
+}
+extern "C" int wf_solver_stats(const wf_solver *h, long *pressure_iterations, long *steps, double *last_error) {
+    WF_DISPATCH(h, stats, pressure_iterations, steps, last_error);
+}
+extern "C" int wf_solver_profile(wf_solver *h, int enable) { WF_DISPATCH(h, profile, enable); }
+extern "C" int wf_solver_kernel_times(const wf_solver *h, int kind, double *total_ms, long *launches) {
+    WF_DISPATCH(h, kernel_times, kind, total_ms, launches);
+}
+extern "C" int wf_solver_last_pressure(const wf_solver *h, double *error, int *iterations) {
+    WF_DISPATCH(h, last_pressure, error, iterations);
+}
+extern "C" int wf_solver_set_closure(wf_solver *h, const float *params, long n) { WF_DISPATCH(h, set_closure, params, n); }
+
+extern "C" int wf_device_synchronize(void) {
+    WF_HIP(hipDeviceSynchronize());
+    return WF_OK;
+}
+
+/* ------------------------------------------------------- learned closure ---- */
+
+static_assert(WF_CL_NPARAMS == WF_CLOSURE_NPARAMS, "closure parameter block differs from the public layout");
+
+/* float -> bf16 bits, round to nearest even (NaN kept quiet) */
+uint16_t wf_bf16_bits(float f) {
+    uint32_t u;
+    memcpy(&u, &f, 4);
+    if ((u & 0x7f800000u) == 0x7f800000u && (u & 0x007fffffu)) return (uint16_t)((u >> 16) | 0x40);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+
+/* The MFMA B-operand fragments of the three layers (layout in wf_closure.h). */
+static void wf_closure_fragments(const float *prm, std::vector<uint16_t> &frag) {
+    frag.assign((size_t)WF_CL_NFRAG * 8, 0);
+    for (int l = 0; l < 64; l++) {
+        const int lr = l & 15, lg = l >> 4;
+        for (int j = 0; j < 8; j++) {
+            for (int nt = 0; nt < 4; nt++) {
+                const int n = nt * 16 + lr, k = 8 * lg + j;
+                const float w1 = (k < WF_CL_NIN) ? prm[WF_CL_W1 + n * WF_CL_NIN + k] : 0.0f;
+                frag[((size_t)(WF_CL_FRAG_W1 + nt * 64 + l)) * 8 + j] = wf_bf16_bits(w1);
+                for (int ks = 0; ks < 2; ks++) {
+                    const int k2 = ks * 32 + 8 * lg + j;
+                    frag[((size_t)(WF_CL_FRAG_W2 + (nt * 2 + ks) * 64 + l)) * 8 + j] =
+                        wf_bf16_bits(prm[WF_CL_W2 + n * WF_CL_NH + k2]);
+                }
+            }
+            for (int ks = 0; ks < 2; ks++) {
+                const int k3 = ks * 32 + 8 * lg + j;
+                frag[((size_t)(WF_CL_FRAG_W3 + ks * 64 + l)) * 8 + j] =
+                    wf_bf16_bits(lr == 0 ? prm[WF_CL_W3 + k3] : 0.0f);
+            }
+        }
+    }
+}
+
+int wf_closure_upload(const float *params, long n, float **prm_dev, void **frag_dev) {
+    if (n != WF_CL_NPARAMS) return wf_fail(WF_E_ARG, "closure: %ld parameters given, %d expected", n, WF_CL_NPARAMS);
+    for (long q = 0; q < n; q++)
+        if (!std::isfinite(params[q])) return wf_fail(WF_E_ARG, "closure: parameter %ld is not finite", q);
+    std::vector<uint16_t> frag;
+    wf_closure_fragments(params, frag);
+    WF_HIP(hipMalloc((void **)prm_dev, sizeof(float) * n));
+    WF_HIP(hipMemcpy(*prm_dev, params, sizeof(float) * n, hipMemcpyHostToDevice));
+    WF_HIP(hipMalloc(frag_dev, sizeof(wf64::wf_bf16x8) * WF_CL_NFRAG));
+    WF_HIP(hipMemcpy(*frag_dev, frag.data(), sizeof(wf64::wf_bf16x8) * WF_CL_NFRAG, hipMemcpyHostToDevice));
+    return WF_OK;
+}
+
+extern "C" int wf_closure_eval(const float *params, long nparams, const double *features, long n, double *nu_t,
+                               int *learned) {
+    if (params == NULL || features == NULL || nu_t == NULL || learned == NULL || n < 0)
+        return wf_fail(WF_E_ARG, "null or negative argument");
+    if (n == 0) return WF_OK;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return wf_fail(WF_E_DEVICE, "no HIP device");
+    WF_HIP(hipSetDevice(0));
+    float *prm = NULL;
+    void *frag = NULL;
+    double *f = NULL, *nu = NULL;
+    int *lf = NULL;
+    int rc = wf_closure_upload(params, nparams, &prm, &frag);
+    hipError_t e = hipSuccess;
+    if (rc == WF_OK) {
+        e = hipMalloc((void **)&f, sizeof(double) * WF_CL_NIN * n);
+        if (e == hipSuccess) e = hipMalloc((void **)&nu, sizeof(double) * n);
+        if (e == hipSuccess) e = hipMalloc((void **)&lf, sizeof(int) * n);
+        if (e == hipSuccess) e = hipMemcpy(f, features, sizeof(double) * WF_CL_NIN * n, hipMemcpyHostToDevice);
+        if (e == hipSuccess)
+            e = wf64::wf_launch_closure_eval(wf64::WfClosure{prm, (const wf64::wf_bf16x8 *)frag}, f, n, nu, lf, 0);
+        if (e == hipSuccess) e = hipMemcpy(nu_t, nu, sizeof(double) * n, hipMemcpyDeviceToHost);
+        if (e == hipSuccess) e = hipMemcpy(learned, lf, sizeof(int) * n, hipMemcpyDeviceToHost);
+        if (e != hipSuccess) rc = wf_fail(WF_E_DEVICE, "closure evaluation: %s", hipGetErrorString(e));
+    }
+    void *bufs[] = {prm, frag, f, nu, lf};
+    for (void *b : bufs)
+        if (b) (void)hipFree(b);
+    return rc;
+}

@@ -31,6 +31,8 @@
 
 #include "wf_common.h"
 
+namespace WF_NS {
+
 #define WF_CL_NIN 12
 #define WF_CL_NH 64
 /* parameter block (floats), the layout of wf_solver_set_closure */
@@ -88,14 +90,14 @@ __device__ __forceinline__ void wf_cl_wave_sync() {
 }
 __device__ __forceinline__ float wf_cl_softplus(float z) { return (z > 20.0f) ? z : log1pf(expf(z)); }
 
-/* Normalised input of one node (FP64 arithmetic, rounded to FP32 then bf16)
- * and whether all raw inputs lie in the training range. */
-__device__ __forceinline__ bool wf_cl_inputs(const float *prm, const double f[WF_CL_NIN], __bf16 x[WF_CL_NIN]) {
+/* Normalised input of one node (wf_real arithmetic, rounded to FP32 then
+ * bf16) and whether all raw inputs lie in the training range. */
+__device__ __forceinline__ bool wf_cl_inputs(const float *prm, const wf_real f[WF_CL_NIN], __bf16 x[WF_CL_NIN]) {
     bool in = true;
 #pragma unroll
     for (int i = 0; i < WF_CL_NIN; i++) {
-        in = in && (f[i] >= (double)prm[WF_CL_LO + i]) && (f[i] <= (double)prm[WF_CL_HI + i]);
-        x[i] = (__bf16)(float)((f[i] - (double)prm[WF_CL_MEAN + i]) * (double)prm[WF_CL_ISTD + i]);
+        in = in && (f[i] >= (wf_real)prm[WF_CL_LO + i]) && (f[i] <= (wf_real)prm[WF_CL_HI + i]);
+        x[i] = (__bf16)(float)((f[i] - (wf_real)prm[WF_CL_MEAN + i]) * (wf_real)prm[WF_CL_ISTD + i]);
     }
     return in;
 }
@@ -175,20 +177,22 @@ __device__ __forceinline__ float wf_cl_forward(const WfClosure &cl, WfClosureLds
 /* nu_t from the network output, and the node's mu_sgs under the closure:
  * -rho nu_t when the inputs are in range and nu_t is finite, else the
  * Smagorinsky value mu_smag (>= 0). */
-__device__ __forceinline__ double wf_cl_mu(const float *prm, bool in_range, float z, double rho, double mu_smag) {
+__device__ __forceinline__ wf_real wf_cl_mu(const float *prm, bool in_range, float z, wf_real rho, wf_real mu_smag) {
     const float nu = prm[WF_CL_SCALE] * wf_cl_softplus(z);
     const bool ok = in_range && (nu >= 0.0f) && (nu < 3.0e38f);
-    return ok ? -(rho * (double)nu) : mu_smag;
+    return ok ? -(rho * (wf_real)nu) : mu_smag;
 }
 
 /* Viscosity the stage pass uses at a point: a learned value (stored
  * negative) is taken as it is, a Smagorinsky value is damped by fw^2. */
 template <bool CL>
-__device__ __forceinline__ double wf_cl_damped(double mu, double fw) {
+__device__ __forceinline__ wf_real wf_cl_damped(wf_real mu, wf_real fw) {
     if constexpr (CL)
-        return (mu < 0.0) ? -mu : mu * (fw * fw);
+        return (mu < WR(0.0)) ? -mu : mu * (fw * fw);
     else
         return mu * (fw * fw);
 }
+
+}  // namespace WF_NS
 
 #endif /* WF_CLOSURE_H */

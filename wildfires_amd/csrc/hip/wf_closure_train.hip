@@ -28,6 +28,8 @@
 #include "wf_closure.h"
 #include "wf_kernels.h"
 
+using namespace WF_NS; /* compiled once, in the FP64 namespace */
+
 #define WF_CLT_NP (WF_CL_NPARAMS + 1) /* gradient entries + the loss */
 #define WF_CLT_MAXWG 1024
 

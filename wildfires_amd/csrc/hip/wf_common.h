@@ -7,9 +7,11 @@
  * The reference layout (z fastest) exists only at the C-ABI boundary and is
  * converted by the layout kernels.
  *
- * FP64 throughout.  Kernels are compiled without FMA contraction, so an
- * expression evaluates in the order written; the stencil passes and the FFT
- * opt in to contraction and take constant divisors as reciprocals where noted.
+ * Precision: wf_real (wf_prec.h) is FP64 in the wf64 compilation and FP32 in
+ * the mixed wf32 one; `double` is FP64 in both.  Kernels are compiled without
+ * FMA contraction, so an expression evaluates in the order written; the
+ * stencil passes and the FFT opt in to contraction and take constant divisors
+ * as reciprocals where noted.
  */
 #ifndef WF_COMMON_H
 #define WF_COMMON_H
@@ -17,29 +19,35 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "wf_prec.h"
+
 #define WF_SIGMA_D 5.670374419e-8
 #define WF_A_VD_D 26.0
+
+namespace WF_NS {
 
 /* Scalar constants and grid tables every kernel needs; passed by value. */
 struct WfConst {
     int Nx, Ny, Nz, Nzm;      /* Nzm = Nz_Y_max */
     long plane;               /* Nx*Ny */
     long N;                   /* Nx*Ny*Nz */
-    double dx, dy, dt;
-    double mu, kT, Y_f, H_C, A, T_act, T_pc, h_c, alpha_s, sigma_s;
-    double T_inf, c_p, rho_inf, C_d, g, delta, Pr;
-    double T_min, T_max, Y_min, Y_max;
-    double u_dead, v_dead, w_dead, T_dead, Y_dead;
+    wf_real dx, dy, dt;
+    wf_real mu, kT, Y_f, H_C, A, T_act, T_pc, h_c, alpha_s, sigma_s;
+    wf_real T_inf, c_p, rho_inf, C_d, g, delta, Pr;
+    wf_real T_min, T_max;
+    double Y_min, Y_max; /* Y is FP64 in both precisions */
+    wf_real u_dead, v_dead, w_dead, T_dead;
+    double Y_dead;
     int temperature_convection, suth_mu, suth_k, variable_density, fd_z;
     int exchange_on; /* alpha_s * sigma_s != 0: the h_c exchange term is evaluated */
     /* reciprocals of the constant divisors of the stencil passes */
-    double inv2dx, inv2dy, invdx2, invdy2; /* 1/(2dx), 1/(2dy), 1/dx^2, 1/dy^2 */
-    double inv_cp, inv_Trho, inv_mu, inv_Pr; /* 1/c_p, 1/(T_inf rho_inf), 1/mu, 1/Pr */
-    const double *zc;      /* WF_ZC z-stencil coefficients per plane (WF_Z_*) */
-    const double *z;       /* Nz */
-    const double *cd2;     /* Nz: (C_s (dx dy dz_k)^(1/3))^2 evaluated on the host */
-    const double *delta_k; /* Nz: filter width (dx dy dz_k)^(1/3), an input of the learned closure */
-    const double *topo;    /* plane */
+    wf_real inv2dx, inv2dy, invdx2, invdy2; /* 1/(2dx), 1/(2dy), 1/dx^2, 1/dy^2 */
+    wf_real inv_cp, inv_Trho, inv_mu, inv_Pr; /* 1/c_p, 1/(T_inf rho_inf), 1/mu, 1/Pr */
+    const wf_real *zc;      /* WF_ZC z-stencil coefficients per plane (WF_Z_*) */
+    const wf_real *z;       /* Nz */
+    const wf_real *cd2;     /* Nz: (C_s (dx dy dz_k)^(1/3))^2 evaluated on the host */
+    const wf_real *delta_k; /* Nz: filter width (dx dy dz_k)^(1/3), an input of the learned closure */
+    const wf_real *topo;    /* plane */
     const int *cut;        /* plane */
     const int *nzy;        /* plane */
 };
@@ -61,17 +69,18 @@ struct WfConst {
 #define WF_Z_D2F 10
 #define WF_Z_D2B 14
 
-/* Device view of a state vector [u|v|w|T|Y] in the internal layout. */
+/* Device view of a state vector [u|v|w|T|Y] in the internal layout (Y FP64). */
 struct WfState {
-    double *u, *v, *w, *T, *Y;
+    wf_real *u, *v, *w, *T;
+    double *Y;
 };
 
 /* Fields the RHS pass hands to its SGS divergence: the reference's
  * R_turbulence planes that are read at a neighbour (src/cu/turbulence.cu:318-342). */
 struct WfDerived {
-    double *mu, *s11, *s12, *s13, *s22, *s23, *s33, *divU;
-    double *utau; /* plane */
-    double *col;  /* 4 x 5 planes: raw RHS (u,v,w,T,Y) at cut+1, cut+2, Nz-2, Nz-3 of each column */
+    wf_real *mu, *s11, *s12, *s13, *s22, *s23, *s33, *divU;
+    wf_real *utau; /* plane */
+    wf_real *col;  /* 4 x 5 planes: raw RHS (u,v,w,T,Y) at cut+1, cut+2, Nz-2, Nz-3 of each column */
 };
 
 __device__ __forceinline__ long wf_q(const WfConst &c, int i, int j, int k) {
@@ -85,25 +94,27 @@ __device__ __forceinline__ int wf_wrap_p(int i, int d, int n) { return (i + d >=
 
 /* Second-order one-sided first derivative from nodes k, k+1, k+2
  * (h0 = z[k+1]-z[k], h1 = z[k+2]-z[k+1]); src/cu/pde.cu:363-365. */
-__device__ __forceinline__ double wf_d1f(double f0, double f1, double f2, double h0, double h1) {
+__device__ __forceinline__ wf_real wf_d1f(wf_real f0, wf_real f1, wf_real f2, wf_real h0, wf_real h1) {
     return -(2 * h0 + h1) * f0 / (h0 * (h0 + h1)) + (h0 + h1) * f1 / (h0 * h1) - h0 * f2 / (h1 * (h0 + h1));
 }
 
 /* Second-order one-sided first derivative from nodes k-2, k-1, k
  * (hm2 = z[k-1]-z[k-2], hm1 = z[k]-z[k-1]); src/cu/pde.cu:383-385. */
-__device__ __forceinline__ double wf_d1b(double fm2, double fm1, double f0, double hm2, double hm1) {
+__device__ __forceinline__ wf_real wf_d1b(wf_real fm2, wf_real fm1, wf_real f0, wf_real hm2, wf_real hm1) {
     return hm1 * fm2 / (hm2 * (hm2 + hm1)) - (hm2 + hm1) * fm1 / (hm2 * hm1) + (hm2 + 2 * hm1) * f0 / (hm1 * (hm2 + hm1));
 }
 
 /* Four-point second derivatives at the bottom / top nodes; src/cu/pde.cu:497-500, 539-542. */
-__device__ __forceinline__ double wf_d2f(double f0, double f1, double f2, double f3, double h0, double h1, double h2) {
+__device__ __forceinline__ wf_real wf_d2f(wf_real f0, wf_real f1, wf_real f2, wf_real f3, wf_real h0, wf_real h1,
+                                          wf_real h2) {
     return (6 * h0 + 4 * h1 + 2 * h2) * f0 / (h0 * (h0 + h1) * (h0 + h1 + h2)) -
            (4 * h0 + 4 * h1 + 2 * h2) * f1 / (h0 * h1 * (h1 + h2)) +
            (4 * h0 + 2 * h1 + 2 * h2) * f2 / (h1 * h2 * (h0 + h1)) -
            (4 * h0 + 2 * h1) * f3 / (h2 * (h1 + h2) * (h0 + h1 + h2));
 }
 
-__device__ __forceinline__ double wf_d2b(double fm3, double fm2, double fm1, double f0, double hm3, double hm2, double hm1) {
+__device__ __forceinline__ wf_real wf_d2b(wf_real fm3, wf_real fm2, wf_real fm1, wf_real f0, wf_real hm3, wf_real hm2,
+                                          wf_real hm1) {
     return -(2 * hm2 + 4 * hm1) * fm3 / (hm3 * (hm3 + hm2) * (hm3 + hm2 + hm1)) +
            (2 * hm3 + 2 * hm2 + 4 * hm1) * fm2 / (hm3 * hm2 * (hm2 + hm1)) -
            (2 * hm3 + 4 * hm2 + 4 * hm1) * fm1 / (hm2 * hm1 * (hm3 + hm2)) +
@@ -111,35 +122,35 @@ __device__ __forceinline__ double wf_d2b(double fm3, double fm2, double fm1, dou
 }
 
 /* Interior second derivative on a non-uniform grid; src/cu/pde.cu:572. */
-__device__ __forceinline__ double wf_d2m(double fm1, double f0, double fp1, double hm1, double h0) {
+__device__ __forceinline__ wf_real wf_d2m(wf_real fm1, wf_real f0, wf_real fp1, wf_real hm1, wf_real h0) {
     return 2 * fm1 / (hm1 * (hm1 + h0)) - 2 * f0 / (hm1 * h0) + 2 * fp1 / (h0 * (hm1 + h0));
 }
 
 /* Arrhenius rate A exp(-T_act/T) H(T - T_pc).  The reference multiplies by the
  * step function; for T <= T_pc that product is +0, which is returned without
  * evaluating exp. */
-__device__ __forceinline__ double wf_rate(const WfConst &c, double T) {
-    return (T > c.T_pc) ? c.A * exp(-c.T_act / T) * 1.0 : 0.0;
+__device__ __forceinline__ wf_real wf_rate(const WfConst &c, wf_real T) {
+    return (T > c.T_pc) ? c.A * exp(-c.T_act / T) * WR(1.0) : WR(0.0);
 }
 
 /* Van Driest damping 1 - exp(-z_ibm u_tau / A_VD / nu).  For an exponent below
  * -38, exp() < 2^-54 and 1 - exp() rounds to 1 exactly, so exp is skipped. */
-__device__ __forceinline__ double wf_fw(double zibm, double utau, double nu) {
-    double a = -zibm * utau / WF_A_VD_D / nu;
-    return (a < -38.0) ? 1.0 : 1 - exp(a);
+__device__ __forceinline__ wf_real wf_fw(wf_real zibm, wf_real utau, wf_real nu) {
+    wf_real a = -zibm * utau / WR(WF_A_VD_D) / nu;
+    return (a < WR(-38.0)) ? WR(1.0) : 1 - exp(a);
 }
 /* The same with the reciprocal of nu. */
-__device__ __forceinline__ double wf_fw_inv(double zibm, double utau, double inv_nu) {
-    double a = -zibm * utau * (1.0 / WF_A_VD_D) * inv_nu;
-    return (a < -38.0) ? 1.0 : 1 - exp(a);
+__device__ __forceinline__ wf_real wf_fw_inv(wf_real zibm, wf_real utau, wf_real inv_nu) {
+    wf_real a = -zibm * utau * WR(1.0 / WF_A_VD_D) * inv_nu;
+    return (a < WR(-38.0)) ? WR(1.0) : 1 - exp(a);
 }
 
 /* Distance to the terrain max(z_k - topo, 0) (src/cu/ibm.cu:84). */
-__device__ __forceinline__ double wf_zibm_t(const WfConst &c, double topo, int k) {
-    double d = c.z[k] - topo;
-    return (d > 0.0) ? d : 0.0;
+__device__ __forceinline__ wf_real wf_zibm_t(const WfConst &c, wf_real topo, int k) {
+    wf_real d = c.z[k] - topo;
+    return (d > WR(0.0)) ? d : WR(0.0);
 }
-__device__ __forceinline__ double wf_zibm(const WfConst &c, int col, int k) { return wf_zibm_t(c, c.topo[col], k); }
+__device__ __forceinline__ wf_real wf_zibm(const WfConst &c, int col, int k) { return wf_zibm_t(c, c.topo[col], k); }
 
 /* Workgroups are dispatched round-robin over the 8 XCDs (each with its own
  * L2); this maps the hardware block index to a logical tile so that each XCD
@@ -207,5 +218,7 @@ __device__ __forceinline__ void wf_tile_ijk(const WfConst &c, int *i, int *j, in
 }
 
 #define WF_CHECK_LAUNCH() (hipGetLastError())
+
+}  // namespace WF_NS
 
 #endif

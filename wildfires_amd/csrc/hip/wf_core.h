@@ -1,0 +1,42 @@
+/*
+ * wf_core.h -- the solver core of each precision (wf_solver.hip compiled as
+ * namespace wf64 and as namespace wf32) and the library-wide helpers of
+ * wf_api.hip, which implements the C-ABI over both.
+ */
+#ifndef WF_CORE_H
+#define WF_CORE_H
+
+#include <stdint.h>
+
+#include "../../../include/wildfires.h"
+
+/* error state of the library (thread-local message, see wf_last_error) */
+int wf_fail(int code, const char *fmt, ...);
+const char *wf_error_message(void);
+void wf_error_restore(const char *msg);
+
+/* the learned closure's FP32 block and MFMA fragments, uploaded to the device */
+int wf_closure_upload(const float *params, long n, float **prm_dev, void **frag_dev);
+
+#define WF_CORE_DECLARE(NS)                                                                                     \
+    namespace NS {                                                                                              \
+    struct Solver;                                                                                              \
+    int create(const Parameters *P, Solver **out);                                                             \
+    int destroy(Solver *s);                                                                                     \
+    int set_state(Solver *s, const double *y_n, const double *p);                                              \
+    int get_state(Solver *s, double *y_n, double *p);                                                          \
+    int step(Solver *s, int step0, int n_steps);                                                               \
+    int phi(Solver *s, const double *R_old, double *R_new);                                                    \
+    int pressure(Solver *s, const double *y_np1, const double *y_n, double *p, double *error, int *iterations); \
+    int project(Solver *s, double *y_np1, const double *y_n, const double *p, int n);                          \
+    int stats(const Solver *s, long *pressure_iterations, long *steps, double *last_error);                   \
+    int profile(Solver *s, int enable);                                                                        \
+    int kernel_times(const Solver *s, int kind, double *total_ms, long *launches);                            \
+    int last_pressure(const Solver *s, double *error, int *iterations);                                       \
+    int set_closure(Solver *s, const float *params, long n);                                                   \
+    }
+
+WF_CORE_DECLARE(wf64)
+WF_CORE_DECLARE(wf32)
+
+#endif

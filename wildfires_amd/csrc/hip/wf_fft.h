@@ -21,9 +21,21 @@
 
 #define WF_MAX_RADICES 16
 
+/* wfc: a complex value in the precision of the compilation (wf_prec.h); the
+ * C check of the plan (no HIP) uses FP64 */
+#if defined(__HIPCC__)
+#include "wf_prec.h"
+namespace WF_NS {
+typedef struct {
+    wf_real re, im;
+} wfc;
+#else
+typedef double wf_real;
+#define WR(x) (x)
 typedef struct {
     double re, im;
 } wfc;
+#endif
 
 /* Output element o of one Stockham stage over a sequence of length n stored at
  * `in`; tw[e] = exp(sign 2 pi i e / n). */
@@ -35,7 +47,7 @@ WF_HD wfc wf_stockham_output(const wfc *in, int n, int R, int Ns, const wfc *tw,
     int m = n / R;
     int scale = n / (Ns * R);
     int step = ((kq + q * Ns) * scale) % n;
-    double ar = 0.0, ai = 0.0;
+    wf_real ar = WR(0.0), ai = WR(0.0);
     int e = 0;
     for (int r = 0; r < R; r++) {
         wfc x = in[jb + r * m];
@@ -54,10 +66,10 @@ WF_HD wfc wf_stockham_output(const wfc *in, int n, int R, int Ns, const wfc *tw,
 /* Spectra of two real sequences a, b packed as z = a + i b, at bin r:
  * A[r] = (Z[r] + conj Z[n-r]) / 2, B[r] = (Z[r] - conj Z[n-r]) / (2i). */
 WF_HD void wf_unpack_pair(wfc zr, wfc zm, wfc *A, wfc *B) {
-    A->re = 0.5 * (zr.re + zm.re);
-    A->im = 0.5 * (zr.im - zm.im);
-    B->re = 0.5 * (zr.im + zm.im);
-    B->im = 0.5 * (zm.re - zr.re);
+    A->re = WR(0.5) * (zr.re + zm.re);
+    A->im = WR(0.5) * (zr.im - zm.im);
+    B->re = WR(0.5) * (zr.im + zm.im);
+    B->im = WR(0.5) * (zm.re - zr.re);
 }
 
 /* Bin r of Z = A + i B for the inverse of two half spectra (r < n/2+1 read
@@ -178,15 +190,15 @@ __device__ __forceinline__ void wf_stage_ip(wfc *buf, const wfc *tw) {
                 const int kq = j % NS;
                 const wfc *src = buf + s * N + j;
                 const wfc x0 = src[0];
-                double c0r = x0.re, c0i = x0.im;
-                double cr[QN], ci[QN], sr[QN], si[QN];
+                wf_real c0r = x0.re, c0i = x0.im;
+                wf_real cr[QN], ci[QN], sr[QN], si[QN];
                 int m[QN]; /* (r q) mod R for this thread's q = 1 + u + qi QS */
 #pragma unroll
                 for (int qi = 0; qi < QN; qi++) {
                     cr[qi] = x0.re;
                     ci[qi] = x0.im;
-                    sr[qi] = 0.0;
-                    si[qi] = 0.0;
+                    sr[qi] = WR(0.0);
+                    si[qi] = WR(0.0);
                     m[qi] = 0;
                 }
                 constexpr int UNR = (H <= 8) ? H : 4;
@@ -197,8 +209,8 @@ __device__ __forceinline__ void wf_stage_ip(wfc *buf, const wfc *tw) {
                         xa = wf_cmul(xa, tw[kq * r * SC]);
                         xb = wf_cmul(xb, tw[kq * (R - r) * SC]);
                     }
-                    const double ar = xa.re + xb.re, ai = xa.im + xb.im;
-                    const double br = xa.re - xb.re, bi = xa.im - xb.im;
+                    const wf_real ar = xa.re + xb.re, ai = xa.im + xb.im;
+                    const wf_real br = xa.re - xb.re, bi = xa.im - xb.im;
                     c0r += ar;
                     c0i += ai;
 #pragma unroll
@@ -338,7 +350,7 @@ static_assert(wf_f2_threads(255) <= 256 && wf_f2_threads(63) <= 256, "two-factor
  * handed to out(k, X_k) in the order 0, 1, R-1, 2, R-2, ... (pairs q, R-q
  * share their sums: X_q, X_{R-q} = x_0 + sum a_r Re W^{rq} +- i sum b_r Im W^{rq}). */
 template <int R, int TS, int SIGN, class Out>
-__device__ __forceinline__ void wf_dft_reg(const wfc *x, const double2 *tw, Out &&out) {
+__device__ __forceinline__ void wf_dft_reg(const wfc *x, const wf_real2 *tw, Out &&out) {
 #pragma clang fp contract(fast)
     constexpr int H = (R - 1) / 2;
     wfc a[H + 1], b[H + 1];
@@ -355,11 +367,11 @@ __device__ __forceinline__ void wf_dft_reg(const wfc *x, const double2 *tw, Out 
     out(0, y0);
 #pragma unroll
     for (int q = 1; q <= H; q++) {
-        double cr = x[0].re, ci = x[0].im, sr = 0.0, si = 0.0;
+        wf_real cr = x[0].re, ci = x[0].im, sr = WR(0.0), si = WR(0.0);
 #pragma unroll
         for (int r = 1; r <= H; r++) {
-            const double2 W = tw[((r * q) % R) * TS];
-            const double c = W.x, sn = (SIGN < 0) ? W.y : -W.y;
+            const wf_real2 W = tw[((r * q) % R) * TS];
+            const wf_real c = W.x, sn = (SIGN < 0) ? W.y : -W.y;
             cr += a[r].re * c;
             ci += a[r].im * c;
             sr += b[r].im * sn;
@@ -378,12 +390,12 @@ __device__ __forceinline__ void wf_dft_reg(const wfc *x, const double2 *tw, Out 
 /* Phase A for sequence s, n1 = this thread's row: load(s, n) gives input n;
  * Y is written to buf[s N + n1 + N1 k2]. */
 template <int N1, int N2, int SIGN, class Load>
-__device__ __forceinline__ void wf_f2_phase_a(wfc *buf, const double2 *tw, int s, int n1, Load &&load) {
+__device__ __forceinline__ void wf_f2_phase_a(wfc *buf, const wf_real2 *tw, int s, int n1, Load &&load) {
 #pragma clang fp contract(fast)
     constexpr int N = N1 * N2;
     /* this row's twiddles W_N^{n1 k2} (n1 k2 < N) are loaded together with
      * its inputs, not one by one where they are used */
-    double2 wt[N2];
+    wf_real2 wt[N2];
 #pragma unroll
     for (int k2 = 1; k2 < N2; k2++) wt[k2] = tw[n1 * k2];
     wfc x[N2];
@@ -392,8 +404,8 @@ __device__ __forceinline__ void wf_f2_phase_a(wfc *buf, const double2 *tw, int s
     wfc *row = buf + s * N + n1;
     wf_dft_reg<N2, N1, SIGN>(x, tw, [&](int k2, wfc y) {
         if (k2 != 0) {
-            const double2 W = wt[k2];
-            const double wi = (SIGN < 0) ? W.y : -W.y;
+            const wf_real2 W = wt[k2];
+            const wf_real wi = (SIGN < 0) ? W.y : -W.y;
             wfc t;
             t.re = y.re * W.x - y.im * wi;
             t.im = y.re * wi + y.im * W.x;
@@ -405,7 +417,7 @@ __device__ __forceinline__ void wf_f2_phase_a(wfc *buf, const double2 *tw, int s
 
 /* Phase B for sequence s, k2 = this thread's column: X[k2 + N2 k1] -> out(k1, X). */
 template <int N1, int N2, int SIGN, class Out>
-__device__ __forceinline__ void wf_f2_phase_b(const wfc *buf, const double2 *tw, int s, int k2, Out &&out) {
+__device__ __forceinline__ void wf_f2_phase_b(const wfc *buf, const wf_real2 *tw, int s, int k2, Out &&out) {
     constexpr int N = N1 * N2;
     wfc y[N1];
     const wfc *col = buf + s * N + N1 * k2;
@@ -413,6 +425,10 @@ __device__ __forceinline__ void wf_f2_phase_b(const wfc *buf, const double2 *tw,
     for (int n1 = 0; n1 < N1; n1++) y[n1] = col[n1];
     wf_dft_reg<N1, N2, SIGN>(y, tw, out);
 }
+#endif
+
+#if defined(__HIPCC__)
+}  // namespace WF_NS
 #endif
 
 #endif /* WF_FFT_H */

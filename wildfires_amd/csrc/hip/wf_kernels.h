@@ -8,6 +8,8 @@
 #include "wf_common.h"
 #include "wf_closure.h"
 
+namespace WF_NS {
+
 /* k_stage modes (RK combination written by the stage pass) */
 #define WF_MODE_PHI 0   /* out = k_s */
 #define WF_MODE_FIRST 1 /* acc = k_1, out = y_n + a k_1 */
@@ -17,10 +19,10 @@
 
 /* Thermal conductivity by Sutherland's law and its T-derivative
  * (src/cu/pde.cu:692-695). */
-__device__ __forceinline__ void wf_sutherland_k(double T, double *kT, double *dk) {
-    *kT = 0.0241 * pow(T / 273.15, 1.5) * (273.15 + 194) / (T + 194);
-    *dk = 0.0241 * (1.5 * pow(T / 273.15, 0.5) * (273.15 + 194) / (T + 194) -
-                    pow(T / 273.15, 1.5) * (273.15 + 194) / ((T + 194) * (T + 194)));
+__device__ __forceinline__ void wf_sutherland_k(wf_real T, wf_real *kT, wf_real *dk) {
+    *kT = WR(0.0241) * pow(T / WR(273.15), WR(1.5)) * (WR(273.15) + 194) / (T + 194);
+    *dk = WR(0.0241) * (WR(1.5) * pow(T / WR(273.15), WR(0.5)) * (WR(273.15) + 194) / (T + 194) -
+                        pow(T / WR(273.15), WR(1.5)) * (WR(273.15) + 194) / ((T + 194) * (T + 194)));
 }
 
 /* Heat source q of the temperature RHS (src/cu/pde.cu:663-699): combustion
@@ -28,51 +30,53 @@ __device__ __forceinline__ void wf_sutherland_k(double T, double *kT, double *dk
  * of the fuel zone k <= Nz_Y.  The exchange term is h_c alpha_s sigma_s (...),
  * identically +-0 when alpha_s sigma_s = 0, in which case it is not formed
  * (q - (+-0) = q).  h_c is evaluated per node from the parameter value.
- * *Yout receives the node's fuel fraction (0 outside the stored fuel). */
-__device__ __forceinline__ double wf_heat_source_y(const WfConst &c, int col, int i, int j, int k, double T,
-                                                   double T_kp1, const double *Yf, double *Yout) {
-    double K_T = wf_rate(c, T);
-    double T_gas = c.T_inf, T_solid = c.T_inf, Y = 0.0;
+ * *Yout receives the node's fuel fraction (0 outside the stored fuel; read
+ * from the FP64 Y and used in wf_real arithmetic). */
+__device__ __forceinline__ wf_real wf_heat_source_y(const WfConst &c, int col, int i, int j, int k, wf_real T,
+                                                    wf_real T_kp1, const double *Yf, wf_real *Yout) {
+    wf_real K_T = wf_rate(c, T);
+    wf_real T_gas = c.T_inf, T_solid = c.T_inf, Y = WR(0.0);
     int nzy = c.nzy[col];
     if (k <= nzy) {
         if (k < nzy)
-            Y = Yf[(long)col + c.plane * (long)k];
+            Y = (wf_real)Yf[(long)col + c.plane * (long)k];
         else
             T_gas = T_kp1;
-        if (Y > 0.0)
+        if (Y > WR(0.0))
             T_solid = T;
         else
             T_gas = T;
     }
-    double q = c.H_C * Y * K_T / c.c_p;
+    wf_real q = c.H_C * Y * K_T / c.c_p;
     if (c.exchange_on) {
-        const double *z = c.z;
-        double dz_k = (k < c.Nz - 1) ? z[k + 1] - z[k] : z[k] - z[k - 1];
-        double h_c = c.h_c * dz_k * pow(fabs(T_gas - T_solid) / dz_k, 1.0 / 4.0);
+        const wf_real *z = c.z;
+        wf_real dz_k = (k < c.Nz - 1) ? z[k + 1] - z[k] : z[k] - z[k - 1];
+        wf_real h_c = c.h_c * dz_k * pow(fabs(T_gas - T_solid) / dz_k, WR(1.0 / 4.0));
         q = q - h_c * c.alpha_s * c.sigma_s * (T_gas - T_solid) / (c.c_p * c.T_inf * c.rho_inf / T_gas);
     }
     *Yout = Y;
     return q;
 }
 
-__device__ __forceinline__ double wf_heat_source(const WfConst &c, int col, int i, int j, int k, double T,
-                                                 double T_kp1, const double *Yf) {
-    double Y;
+__device__ __forceinline__ wf_real wf_heat_source(const WfConst &c, int col, int i, int j, int k, wf_real T,
+                                                  wf_real T_kp1, const double *Yf) {
+    wf_real Y;
     return wf_heat_source_y(c, col, i, j, k, T, T_kp1, Yf, &Y);
 }
 
 /* stage pass (wf_stage.hip) */
 hipError_t wf_launch_derived(const WfConst &c, const WfState &X, const WfDerived &D, const WfClosure *cl,
                              hipStream_t s);
-/* library-internal helpers shared by the solver and the closure trainer */
-int wf_set_error(int code, const char *msg);
-uint16_t wf_bf16_bits(float f);
-hipError_t wf_launch_closure_eval(const WfClosure &cl, const double *feat, long n, double *nu, int *learned,
+hipError_t wf_launch_closure_eval(const WfClosure &cl, const wf_real *feat, long n, wf_real *nu, int *learned,
                                   hipStream_t s);
 hipError_t wf_launch_stage(const WfConst &c, const WfState &X, const WfDerived &D, const WfState &Yn,
                            const WfState &acc, const WfState &out, int mode, double a, bool closure, hipStream_t s);
-hipError_t wf_launch_to_internal(const double *src, double *dst, int Nx, int Ny, int nz, hipStream_t s);
-hipError_t wf_launch_to_reference(const double *src, double *dst, int Nx, int Ny, int nz, hipStream_t s);
+/* reference layout (FP64, z fastest) <-> internal layout in type D (wf_real
+ * for u, v, w, T, p; double for Y) */
+template <typename D>
+hipError_t wf_launch_to_internal(const double *src, D *dst, int Nx, int Ny, int nz, hipStream_t s);
+template <typename D>
+hipError_t wf_launch_to_reference(const D *src, double *dst, int Nx, int Ny, int nz, hipStream_t s);
 
 /* pressure pass (wf_pressure.hip) */
 
@@ -106,25 +110,29 @@ struct WfSpectral {
     int fy_cols;    /* kx columns per workgroup in the y-transform */
     int ngroups;    /* row-pair groups per slab in the x-transforms */
     int npmax;      /* workgroups of the inverse x-transform (partial maxima) */
-    double2 *F;     /* [nz][hx][ldj] spectral working buffer (ky fastest) */
-    double2 *Ptop;  /* [hx][ldj] spectrum of the top pressure plane */
-    double *A;      /* Nx*Ny*Nz: p-independent part of the Poisson RHS */
-    const double *UB;    /* [chunk][hx][ldj]: Thomas pivot u_{k0-1} entering each z-chunk (chunk >= 1) */
+    wf_real2 *F;    /* [nz][hx][ldj] spectral working buffer (ky fastest) */
+    wf_real2 *Ptop; /* [hx][ldj] spectrum of the top pressure plane */
+    wf_real *A;     /* Nx*Ny*Nz: p-independent part of the Poisson RHS */
+    const double *UB;    /* [chunk][hx][ldj]: Thomas pivot u_{k0-1} entering each z-chunk (chunk >= 1), FP64 */
     int thomas_lc;       /* z-levels per chunk of the tridiagonal solve */
     double *pmax;        /* npmax partial maxima of |p_new - p_old| */
-    const double *kx2;   /* hx: kx^2 of the retained x-wavenumbers */
-    const double *ky2;   /* ny */
-    const double *ta;    /* nz-1: sub-diagonal a_k (k-only) */
-    const double *tc;    /* nz-1: super-diagonal c_k for k >= 1 (c_0 depends on K^2) */
-    const double2 *twx;  /* nx twiddles exp(-2 pi i e / nx) */
-    const double2 *twy;  /* ny twiddles exp(-2 pi i e / ny) */
+    const wf_real *kx2;  /* hx: kx^2 of the retained x-wavenumbers */
+    const wf_real *ky2;  /* ny */
+    const wf_real *ta;   /* nz-1: sub-diagonal a_k (k-only) */
+    const wf_real *tc;   /* nz-1: super-diagonal c_k for k >= 1 (c_0 depends on K^2) */
+    /* the same tables in FP64, for the systems solved in FP64 (all of them in
+     * the wf64 compilation, the lowest wavenumbers in the mixed one) */
+    const double *kx2d, *ky2d, *tad, *tcd, *betad;
+    int low_kx, low_ky;  /* mixed form: kx < low_kx and min(ky, ny-ky) < low_ky are solved in FP64 */
+    const wf_real2 *twx; /* nx twiddles exp(-2 pi i e / nx) */
+    const wf_real2 *twy; /* ny twiddles exp(-2 pi i e / ny) */
     const int *radx;     /* radices of nx, 0-terminated (<= 16 entries) */
     const int *rady;
     double dz0, top_div, c0a, c0b; /* c_0 = c0a - 0.5 dz0 K^2 - c0b */
     double b0;                     /* b_0 = -1/dz0 */
     double beta_top;               /* 2/(dz_{Nz-2} dz_{Nz-1}) */
-    const double *beta;            /* nz: 2/(dz_{k-1} dz_k) for 1 <= k <= nz-2 */
-    const double *rho;             /* rho^n on every node (variable density) */
+    const wf_real *beta;           /* nz: 2/(dz_{k-1} dz_k) for 1 <= k <= nz-2 */
+    const wf_real *rho;            /* rho^n on every node (variable density) */
     WfIterCtl *ctl;
     int *hflag;                    /* mapped host memory: [0] done, [1] solves finished */
 };
@@ -135,29 +143,35 @@ struct WfSpectral {
 enum { WF_F_ITER = 0, WF_F_FIRST = 1, WF_F_CONST = 2, WF_F_TOP = 3 };
 
 hipError_t wf_launch_pressure_rhs_fx(const WfConst &c, const WfSpectral &sp, const WfState &Ys, const WfState &Yn,
-                                     const double *p, int mode, hipStream_t s);
-hipError_t wf_launch_fy(const WfSpectral &sp, double2 *buf, long slab_stride, int slab0, int nslabs, int sign,
+                                     const wf_real *p, int mode, hipStream_t s);
+hipError_t wf_launch_fy(const WfSpectral &sp, wf_real2 *buf, long slab_stride, int slab0, int nslabs, int sign,
                         int guarded, hipStream_t s);
-hipError_t wf_launch_density(const WfConst &c, const double *T, double *rho, hipStream_t s);
+hipError_t wf_launch_density(const WfConst &c, const wf_real *T, wf_real *rho, hipStream_t s);
 hipError_t wf_launch_iter_reset(WfIterCtl *ctl, hipStream_t s);
 hipError_t wf_launch_thomas(const WfSpectral &sp, hipStream_t s);
 hipError_t wf_launch_pivots(const WfSpectral &sp, double *UB, hipStream_t s);
-hipError_t wf_launch_inverse_fx(const WfConst &c, const WfSpectral &sp, double *p, hipStream_t s);
+hipError_t wf_launch_inverse_fx(const WfConst &c, const WfSpectral &sp, wf_real *p, hipStream_t s);
 hipError_t wf_launch_iter_check(const WfSpectral &sp, double tol, int m, int max_iter, hipStream_t s);
 
 /* Temperature source applied at the end of a step (src/cu/functions.cu:325-403). */
 struct WfSource {
     int mode;  /* 0 none, 1 max(T, T at t_0), 2 delayed shape */
     int shape; /* mode 2: 1 gaussian, 2 parallelepiped, 0 none */
-    double scale, T_inf, T_source;
-    double xc, yc, zc, sx, sy, sz;
-    double x0, x1, y0, y1, z0, z1;
-    const double *x, *y; /* node coordinates */
-    const double *T_src; /* mode 1: T at the start of the run (internal layout) */
+    wf_real scale, T_inf, T_source;
+    wf_real xc, yc, zc, sx, sy, sz;
+    wf_real x0, x1, y0, y1, z0, z1;
+    const wf_real *x, *y; /* node coordinates */
+    const wf_real *T_src; /* mode 1: T at the start of the run (internal layout) */
 };
 
 /* projection pass (wf_project.hip) */
-hipError_t wf_launch_project(const WfConst &c, const WfState &Ys, const WfState &Yn, const double *p,
+hipError_t wf_launch_project(const WfConst &c, const WfState &Ys, const WfState &Yn, const wf_real *p,
                              const WfState &out, const WfSource &src, hipStream_t s);
+
+}  // namespace WF_NS
+
+/* library-wide helpers (wf_api.hip), outside the precision namespaces */
+int wf_set_error(int code, const char *msg);
+uint16_t wf_bf16_bits(float f);
 
 #endif

@@ -29,10 +29,19 @@
 
 #define WF_NT 256
 
+namespace WF_NS {
+
+__device__ __forceinline__ wf_real2 wf_make2(wf_real x, wf_real y) {
+    wf_real2 r;
+    r.x = x;
+    r.y = y;
+    return r;
+}
+
 /* 32-bit element offsets from a uniform base (scalar base + 32-bit lane offset;
  * the solver limits Nx*Ny*Nz to 2^28). */
-__device__ __forceinline__ double wf_ldp(const double *base, unsigned q) {
-    return *(const double *)((const char *)base + (q << 3));
+__device__ __forceinline__ wf_real wf_ldp(const wf_real *base, unsigned q) {
+    return *(const wf_real *)((const char *)base + q * (unsigned)sizeof(wf_real));
 }
 
 /* One full FFT of `nseq` sequences of run-time length n held in LDS
@@ -87,52 +96,52 @@ __device__ __forceinline__ void wf_block_max_store(double v, double *out) {
 /* p-independent part of f at node (i,j,k), 1 <= k <= Nz-2
  * (src/cu/pressure.cu:14-101 for constant density, 103-327 otherwise).
  * [Q1] h_c per node with |T_gas - T_solid|. */
-__device__ double wf_f_fixed(const WfConst &c, const WfState &Ys, const WfState &Yn, int i, int j, int k) {
+__device__ wf_real wf_f_fixed(const WfConst &c, const WfState &Ys, const WfState &Yn, int i, int j, int k) {
     const int Nx = c.Nx, Ny = c.Ny;
     const long P = c.plane;
-    const double *z = c.z;
+    const wf_real *z = c.z;
     int im1 = wf_wrap_m(i, 1, Nx), ip1 = wf_wrap_p(i, 1, Nx);
     int jm1 = wf_wrap_m(j, 1, Ny), jp1 = wf_wrap_p(j, 1, Ny);
     long q = wf_q(c, i, j, k);
     long qxm = wf_q(c, im1, j, k), qxp = wf_q(c, ip1, j, k), qym = wf_q(c, i, jm1, k), qyp = wf_q(c, i, jp1, k);
-    double dz_km1 = z[k] - z[k - 1], dz_k = z[k + 1] - z[k];
-    double u = Ys.u[q], v = Ys.v[q];
-    double wz = (Ys.w[q + P] - Ys.w[q - P]) / (dz_k + dz_km1);
-    double u_iph = 0.5 * (Ys.u[qxp] + u), u_imh = 0.5 * (u + Ys.u[qxm]);
-    double v_jph = 0.5 * (Ys.v[qyp] + v), v_jmh = 0.5 * (v + Ys.v[qym]);
-    double ux = (u_iph - u_imh) / c.dx, vy = (v_jph - v_jmh) / c.dy;
+    wf_real dz_km1 = z[k] - z[k - 1], dz_k = z[k + 1] - z[k];
+    wf_real u = Ys.u[q], v = Ys.v[q];
+    wf_real wz = (Ys.w[q + P] - Ys.w[q - P]) / (dz_k + dz_km1);
+    wf_real u_iph = WR(0.5) * (Ys.u[qxp] + u), u_imh = WR(0.5) * (u + Ys.u[qxm]);
+    wf_real v_jph = WR(0.5) * (Ys.v[qyp] + v), v_jmh = WR(0.5) * (v + Ys.v[qym]);
+    wf_real ux = (u_iph - u_imh) / c.dx, vy = (v_jph - v_jmh) / c.dy;
     if (c.variable_density == 0) return c.rho_inf * (ux + vy + wz) / c.dt;
-    double T = Ys.T[q];
-    double Tip = Ys.T[qxp], Tim = Ys.T[qxm], Tjp = Ys.T[qyp], Tjm = Ys.T[qym];
-    double Tkp = Ys.T[q + P], Tkm = Ys.T[q - P];
-    double rho = c.T_inf * c.rho_inf / Yn.T[q];
-    double Tx = (Tip - Tim) / (2 * c.dx), Ty = (Tjp - Tjm) / (2 * c.dy);
-    double Txx = (Tip - 2 * T + Tim) / (c.dx * c.dx), Tyy = (Tjp - 2 * T + Tjm) / (c.dy * c.dy);
-    double Tz = (Tkp - Tkm) / (dz_k + dz_km1);
-    double Tzz = wf_d2m(Tkm, T, Tkp, dz_km1, dz_k);
-    double lap_T = Txx + Tyy + Tzz;
-    double kT = c.kT, dk = 0.0;
-    double Y = 0.0;
-    double K_T = wf_rate(c, T);
-    double T_gas = T, T_solid = T;
+    wf_real T = Ys.T[q];
+    wf_real Tip = Ys.T[qxp], Tim = Ys.T[qxm], Tjp = Ys.T[qyp], Tjm = Ys.T[qym];
+    wf_real Tkp = Ys.T[q + P], Tkm = Ys.T[q - P];
+    wf_real rho = c.T_inf * c.rho_inf / Yn.T[q];
+    wf_real Tx = (Tip - Tim) / (2 * c.dx), Ty = (Tjp - Tjm) / (2 * c.dy);
+    wf_real Txx = (Tip - 2 * T + Tim) / (c.dx * c.dx), Tyy = (Tjp - 2 * T + Tjm) / (c.dy * c.dy);
+    wf_real Tz = (Tkp - Tkm) / (dz_k + dz_km1);
+    wf_real Tzz = wf_d2m(Tkm, T, Tkp, dz_km1, dz_k);
+    wf_real lap_T = Txx + Tyy + Tzz;
+    wf_real kT = c.kT, dk = WR(0.0);
+    wf_real Y = WR(0.0);
+    wf_real K_T = wf_rate(c, T);
+    wf_real T_gas = T, T_solid = T;
     int col = i + Nx * j;
     int nzy = c.nzy[col];
     if (k <= nzy) {
         if (k < nzy) Y = Ys.Y[q];
-        if (Y > 0.0)
+        if (Y > WR(0.0))
             T_gas = c.T_pc + (T - c.T_pc);
         else
             T_gas = T;
     }
     if (c.suth_k == 1) wf_sutherland_k(T, &kT, &dk);
-    double qs = c.H_C * Y * K_T / c.c_p;
+    wf_real qs = c.H_C * Y * K_T / c.c_p;
     if (c.exchange_on) {
-        double h_c = c.h_c * dz_k * pow(fabs(T_gas - T_solid) / dz_k, 1.0 / 4.0);
+        wf_real h_c = c.h_c * dz_k * pow(fabs(T_gas - T_solid) / dz_k, WR(1.0) / WR(4.0));
         qs = qs - h_c * c.alpha_s * c.sigma_s * (T_gas - T_solid) / (c.c_p * c.T_inf * c.rho_inf / T_gas);
     }
-    double T2 = T * T, T3 = T * T * T;
-    double div_U_temp = (dk + 12 * WF_SIGMA_D * c.delta * T2 * (Tx * Tx + Ty * Ty + Tz * Tz) +
-                         (kT + 4 * WF_SIGMA_D * c.delta * T3) * lap_T) / (c.T_inf * c.rho_inf * c.c_p) + qs / T;
+    wf_real T2 = T * T, T3 = T * T * T;
+    wf_real div_U_temp = (dk + 12 * WR(WF_SIGMA_D) * c.delta * T2 * (Tx * Tx + Ty * Ty + Tz * Tz) +
+                         (kT + 4 * WR(WF_SIGMA_D) * c.delta * T3) * lap_T) / (c.T_inf * c.rho_inf * c.c_p) + qs / T;
     return rho * (ux + vy + wz - div_U_temp) / c.dt;
 }
 
@@ -140,7 +149,7 @@ __device__ double wf_f_fixed(const WfConst &c, const WfState &Ys, const WfState 
  * the density of y_n (src/cu/pressure.cu:163-197, 269-282, 319).  The
  * constant divisors 2dx, 2dy, z[k+1]-z[k-1] are taken as reciprocals and the
  * sum divided by rho once. */
-__device__ __forceinline__ double wf_f_pgrad(const WfConst &c, const double *rho_n, const double *p, int i, int j,
+__device__ __forceinline__ wf_real wf_f_pgrad(const WfConst &c, const wf_real *rho_n, const wf_real *p, int i, int j,
                                              int k) {
 #pragma clang fp contract(fast)
     const int Nx = c.Nx, Ny = c.Ny;
@@ -150,19 +159,19 @@ __device__ __forceinline__ double wf_f_pgrad(const WfConst &c, const double *rho
     const unsigned q = (unsigned)wf_q(c, i, j, k);
     const unsigned qxm = (unsigned)wf_q(c, im1, j, k), qxp = (unsigned)wf_q(c, ip1, j, k);
     const unsigned qym = (unsigned)wf_q(c, i, jm1, k), qyp = (unsigned)wf_q(c, i, jp1, k);
-    const double dzc = c.zc[WF_ZC * k + WF_Z_DZC];
-    const double rho = wf_ldp(rho_n, q);
-    const double rhox = (wf_ldp(rho_n, qxp) - wf_ldp(rho_n, qxm)) * c.inv2dx;
-    const double rhoy = (wf_ldp(rho_n, qyp) - wf_ldp(rho_n, qym)) * c.inv2dy;
-    const double px = (wf_ldp(p, qxp) - wf_ldp(p, qxm)) * c.inv2dx;
-    const double py = (wf_ldp(p, qyp) - wf_ldp(p, qym)) * c.inv2dy;
-    const double rhoz = (wf_ldp(rho_n, q + P) - wf_ldp(rho_n, q - P)) * dzc;
-    const double pz = (wf_ldp(p, q + P) - wf_ldp(p, q - P)) * dzc;
+    const wf_real dzc = c.zc[WF_ZC * k + WF_Z_DZC];
+    const wf_real rho = wf_ldp(rho_n, q);
+    const wf_real rhox = (wf_ldp(rho_n, qxp) - wf_ldp(rho_n, qxm)) * c.inv2dx;
+    const wf_real rhoy = (wf_ldp(rho_n, qyp) - wf_ldp(rho_n, qym)) * c.inv2dy;
+    const wf_real px = (wf_ldp(p, qxp) - wf_ldp(p, qxm)) * c.inv2dx;
+    const wf_real py = (wf_ldp(p, qyp) - wf_ldp(p, qym)) * c.inv2dy;
+    const wf_real rhoz = (wf_ldp(rho_n, q + P) - wf_ldp(rho_n, q - P)) * dzc;
+    const wf_real pz = (wf_ldp(p, q + P) - wf_ldp(p, q - P)) * dzc;
     return (rhox * px + rhoy * py + rhoz * pz) / rho;
 }
 
 /* rho^n = T_inf rho_inf / T^n on every node, once per step. */
-__global__ void k_density(WfConst c, const double *T, double *rho) {
+__global__ void k_density(WfConst c, const wf_real *T, wf_real *rho) {
     long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (q < c.N) rho[q] = c.T_inf * c.rho_inf / T[q];
 }
@@ -222,15 +231,15 @@ __global__ __launch_bounds__(WF_NT) void k_iter_check(WfSpectral sp, double tol,
 
 /* Input value of the x-transform at (i, j, k) for form MODE. */
 template <int MODE>
-__device__ __forceinline__ double wf_fx_input(const WfConst &c, const WfSpectral &sp, const WfState &Ys,
-                                              const WfState &Yn, const double *p, int i, int j, int k) {
+__device__ __forceinline__ wf_real wf_fx_input(const WfConst &c, const WfSpectral &sp, const WfState &Ys,
+                                              const WfState &Yn, const wf_real *p, int i, int j, int k) {
     long q = wf_q(c, i, j, k);
     if constexpr (MODE == WF_F_TOP) {
         return p[q];
     } else if constexpr (MODE == WF_F_CONST) {
         return wf_f_fixed(c, Ys, Yn, i, j, k);
     } else if constexpr (MODE == WF_F_FIRST) {
-        double f = wf_f_fixed(c, Ys, Yn, i, j, k);
+        wf_real f = wf_f_fixed(c, Ys, Yn, i, j, k);
         sp.A[q] = f;
         return f + wf_f_pgrad(c, sp.rho, p, i, j, k);
     } else {
@@ -243,7 +252,7 @@ __device__ __forceinline__ double wf_fx_input(const WfConst &c, const WfSpectral
  * half-spectrum into F[k][kx][j].  NX > 0 selects the compile-time plan. */
 template <int NX, int MODE>
 __global__ __launch_bounds__(WF_NT) void k_rhs_fx(WfConst c, WfSpectral sp, WfState Ys, WfState Yn,
-                                                  const double *p) {
+                                                  const wf_real *p) {
     if (MODE != WF_F_TOP && sp.ctl->done) return;
     extern __shared__ wfc smem[];
     __shared__ int rad[WF_MAX_RADICES];
@@ -270,11 +279,11 @@ __global__ __launch_bounds__(WF_NT) void k_rhs_fx(WfConst c, WfSpectral sp, WfSt
             const int tc = (t < PAIRS * NX) ? t : PAIRS * NX - 1;
             const int s = tc / NX, i = tc - s * NX;
             const int ja = j0 + 2 * s, jb = ja + 1;
-            const double fa = wf_fx_input<MODE>(c, sp, Ys, Yn, p, i, (ja < ny) ? ja : ny - 1, k);
-            const double fb = wf_fx_input<MODE>(c, sp, Ys, Yn, p, i, (jb < ny) ? jb : ny - 1, k);
+            const wf_real fa = wf_fx_input<MODE>(c, sp, Ys, Yn, p, i, (ja < ny) ? ja : ny - 1, k);
+            const wf_real fb = wf_fx_input<MODE>(c, sp, Ys, Yn, p, i, (jb < ny) ? jb : ny - 1, k);
             if (t < PAIRS * NX) {
-                bufa[t].re = (ja < ny) ? fa : 0.0;
-                bufa[t].im = (jb < ny) ? fb : 0.0;
+                bufa[t].re = (ja < ny) ? fa : WR(0.0);
+                bufa[t].im = (jb < ny) ? fb : WR(0.0);
             }
             __builtin_amdgcn_sched_barrier(0);
         }
@@ -286,7 +295,7 @@ __global__ __launch_bounds__(WF_NT) void k_rhs_fx(WfConst c, WfSpectral sp, WfSt
             if (t < PAIRS * NX) {
                 const int s = t / NX, i = t - s * NX;
                 const int ja = j0 + 2 * s, jb = ja + 1;
-                double fa = 0.0, fb = 0.0;
+                wf_real fa = WR(0.0), fb = WR(0.0);
                 if (ja < ny) fa = wf_fx_input<MODE>(c, sp, Ys, Yn, p, i, ja, k);
                 if (jb < ny) fb = wf_fx_input<MODE>(c, sp, Ys, Yn, p, i, jb, k);
                 bufa[t].re = fa;
@@ -297,7 +306,7 @@ __global__ __launch_bounds__(WF_NT) void k_rhs_fx(WfConst c, WfSpectral sp, WfSt
         for (int t = threadIdx.x; t < pairs * nx; t += blockDim.x) {
             const int s = t / nx, i = t - s * nx;
             const int ja = j0 + 2 * s, jb = ja + 1;
-            double fa = 0.0, fb = 0.0;
+            wf_real fa = WR(0.0), fb = WR(0.0);
             if (ja < ny) fa = wf_fx_input<MODE>(c, sp, Ys, Yn, p, i, ja, k);
             if (jb < ny) fb = wf_fx_input<MODE>(c, sp, Ys, Yn, p, i, jb, k);
             bufa[t].re = fa;
@@ -306,7 +315,7 @@ __global__ __launch_bounds__(WF_NT) void k_rhs_fx(WfConst c, WfSpectral sp, WfSt
     }
     __syncthreads();
     wfc *res = wf_transform<NX, PAIRS>(bufa, bufb, nx, pairs, rad, tw);
-    double2 *dst = (MODE == WF_F_TOP) ? sp.Ptop : sp.F + (long)k * hx * ldj;
+    wf_real2 *dst = (MODE == WF_F_TOP) ? sp.Ptop : sp.F + (long)k * hx * ldj;
     const int rows = 2 * pairs;
     for (int t = threadIdx.x; t < rows * hx; t += blockDim.x) {
         const int jl = t % rows, r = t / rows;
@@ -316,24 +325,24 @@ __global__ __launch_bounds__(WF_NT) void k_rhs_fx(WfConst c, WfSpectral sp, WfSt
         const int rm = (r == 0) ? 0 : nx - r;
         wfc A, B;
         wf_unpack_pair(res[s * nx + r], res[s * nx + rm], &A, &B);
-        dst[(long)r * ldj + j] = (jl & 1) ? make_double2(B.re, B.im) : make_double2(A.re, A.im);
+        dst[(long)r * ldj + j] = (jl & 1) ? wf_make2(B.re, B.im) : wf_make2(A.re, A.im);
     }
 }
 
 /* P2/P4: y-transform of sp.fy_cols kx columns (each contiguous over ky) of
  * slab slab0 + blockIdx.y; sign -1 forward, +1 inverse. */
 template <int NY, int SIGN>
-__device__ __forceinline__ void wf_fy_f2(const WfSpectral &sp, double2 *base, wfc *buf, int r0,
-                                         const double2 *__restrict__ twy) {
+__device__ __forceinline__ void wf_fy_f2(const WfSpectral &sp, wf_real2 *base, wfc *buf, int r0,
+                                         const wf_real2 *__restrict__ twy) {
     constexpr int N1 = wf_f2_n1(NY), N2 = wf_f2_n2(NY), NSEQ = wf_f2_nseq(NY);
     const int hx = sp.hx, ldj = sp.ldj;
     const int t = threadIdx.x;
     if (t < N1 * NSEQ) {
         const int s = t / N1, n1 = t - s * N1;
         wf_f2_phase_a<N1, N2, SIGN>(buf, twy, s, n1, [&](int ss, int n) {
-            wfc v = {0.0, 0.0};
+            wfc v = {WR(0.0), WR(0.0)};
             if (r0 + ss < hx) {
-                const double2 x = base[(long)(r0 + ss) * ldj + n];
+                const wf_real2 x = base[(long)(r0 + ss) * ldj + n];
                 v.re = x.x;
                 v.im = x.y;
             }
@@ -344,8 +353,8 @@ __device__ __forceinline__ void wf_fy_f2(const WfSpectral &sp, double2 *base, wf
     if (t < N2 * NSEQ) {
         const int s = t / N2, k2 = t - s * N2;
         if (r0 + s < hx) {
-            double2 *col = base + (long)(r0 + s) * ldj + k2;
-            wf_f2_phase_b<N1, N2, SIGN>(buf, twy, s, k2, [&](int k1, wfc X) { col[N2 * k1] = make_double2(X.re, X.im); });
+            wf_real2 *col = base + (long)(r0 + s) * ldj + k2;
+            wf_f2_phase_b<N1, N2, SIGN>(buf, twy, s, k2, [&](int k1, wfc X) { col[N2 * k1] = wf_make2(X.re, X.im); });
         }
     }
 }
@@ -354,11 +363,11 @@ __device__ __forceinline__ void wf_fy_f2(const WfSpectral &sp, double2 *base, wf
  * that the compiler may keep its loads apart from the column stores (scalar
  * loads for the DFT constants, vector loads issued early for the rest). */
 template <int NY>
-__global__ __launch_bounds__(WF_NT) void k_fy(WfSpectral sp, double2 *buf, long slab_stride, int slab0, int sign,
-                                              int guarded, const double2 *__restrict__ twy) {
+__global__ __launch_bounds__(WF_NT) void k_fy(WfSpectral sp, wf_real2 *buf, long slab_stride, int slab0, int sign,
+                                              int guarded, const wf_real2 *__restrict__ twy) {
     if (guarded && sp.ctl->done) return;
     extern __shared__ wfc smem[];
-    double2 *base = buf + (long)(slab0 + blockIdx.y) * slab_stride;
+    wf_real2 *base = buf + (long)(slab0 + blockIdx.y) * slab_stride;
     const int cols = sp.fy_cols;
     const int r0 = blockIdx.x * cols;
     if constexpr (NY > 0) {
@@ -371,16 +380,16 @@ __global__ __launch_bounds__(WF_NT) void k_fy(WfSpectral sp, double2 *buf, long 
         const int ny = sp.ny, hx = sp.hx, ldj = sp.ldj;
         wfc *tw = smem, *bufa = smem + ny, *bufb = bufa + cols * ny;
         for (int t = threadIdx.x; t < ny; t += blockDim.x) {
-            double2 w = sp.twy[t];
+            wf_real2 w = sp.twy[t];
             tw[t].re = w.x;
             tw[t].im = (sign > 0) ? -w.y : w.y;
         }
         if (threadIdx.x < WF_MAX_RADICES) rad[threadIdx.x] = sp.rady[threadIdx.x];
         for (int t = threadIdx.x; t < cols * ny; t += blockDim.x) {
             const int cc = t / ny, jj = t - cc * ny;
-            wfc v = {0.0, 0.0};
+            wfc v = {WR(0.0), WR(0.0)};
             if (r0 + cc < hx) {
-                double2 x = base[(long)(r0 + cc) * ldj + jj];
+                wf_real2 x = base[(long)(r0 + cc) * ldj + jj];
                 v.re = x.x;
                 v.im = x.y;
             }
@@ -390,22 +399,41 @@ __global__ __launch_bounds__(WF_NT) void k_fy(WfSpectral sp, double2 *buf, long 
         wfc *res = wf_lds_fft(bufa, bufb, ny, cols, rad, tw);
         for (int t = threadIdx.x; t < cols * ny; t += blockDim.x) {
             const int cc = t / ny, jj = t - cc * ny;
-            if (r0 + cc < hx) base[(long)(r0 + cc) * ldj + jj] = make_double2(res[t].re, res[t].im);
+            if (r0 + cc < hx) base[(long)(r0 + cc) * ldj + jj] = wf_make2(res[t].re, res[t].im);
         }
     }
+}
+
+/* Complex value in the arithmetic type TA of a tridiagonal solve. */
+template <typename TA>
+struct alignas(2 * sizeof(TA)) wf_cx {
+    TA x, y;
+};
+/* Entry i of a coefficient table in TA: the FP64 copy for TA = double. */
+template <typename TA>
+__device__ __forceinline__ TA wf_tab(const wf_real *t, const double *td, int i) {
+    if constexpr (sizeof(TA) == sizeof(double))
+        return td[i];
+    else
+        return (TA)t[i];
+}
+template <typename TA>
+__device__ __forceinline__ wf_cx<TA> wf_ldF(const wf_real2 *F, long i) {
+    const wf_real2 v = F[i];
+    return wf_cx<TA>{(TA)v.x, (TA)v.y};
 }
 
 /* Thomas pivots of every (kx, ky) system (src/cu/pressure.cu:429-433):
  * u_0 = b_0, u_k = b_k - (a_{k-1}/u_{k-1}) c_{k-1}.  They do not change between
  * steps; the pivot entering each z-chunk of the tridiagonal solve (u_{k0-1},
- * k0 = chunk * lc) is stored once, and the solve re-forms the others with the
- * same operations. */
-__device__ __forceinline__ double wf_pivot_next(const WfSpectral &sp, double u, int k, double K2, double c0,
-                                                double *l_out) {
+ * k0 = chunk * lc) is stored once (FP64), and the solve re-forms the others
+ * with the same operations in its own arithmetic type TA. */
+template <typename TA>
+__device__ __forceinline__ TA wf_pivot_next(const WfSpectral &sp, TA u, int k, TA K2, TA c0, TA *l_out) {
     const int nz = sp.nz;
-    const double ck1 = (k == 1) ? c0 : sp.tc[k - 1];
-    const double bk = (k == nz - 1) ? -(K2 + sp.beta_top) : -(K2 + sp.beta[k]);
-    const double l = sp.ta[k - 1] / u;
+    const TA ck1 = (k == 1) ? c0 : wf_tab<TA>(sp.tc, sp.tcd, k - 1);
+    const TA bk = (k == nz - 1) ? -(K2 + (TA)sp.beta_top) : -(K2 + wf_tab<TA>(sp.beta, sp.betad, k));
+    const TA l = wf_tab<TA>(sp.ta, sp.tad, k - 1) / u;
     *l_out = l;
     return bk - l * ck1;
 }
@@ -417,30 +445,27 @@ __global__ void k_pivots(WfSpectral sp, double *UB) {
     const int nz = sp.nz, lc = sp.thomas_lc;
     const long slab = (long)sp.hx * sp.ldj;
     const long o = (long)r * sp.ldj + j;
-    const double K2 = sp.kx2[r] + sp.ky2[j];
+    const double K2 = sp.kx2d[r] + sp.ky2d[j];
     const double c0 = sp.c0a - 0.5 * sp.dz0 * K2 - sp.c0b;
     double u = sp.b0;
     for (int k = 1; k < nz; k++) {
         if (k % lc == 0) UB[o + (k / lc) * slab] = u; /* u_{k-1} enters chunk k/lc */
         double l;
-        u = wf_pivot_next(sp, u, k, K2, c0, &l);
+        u = wf_pivot_next<double>(sp, u, k, K2, c0, &l);
     }
 }
 
 /* P3: tridiagonal solve for 64 consecutive ky of one kx column per workgroup,
  * in place in F (src/cu/pressure.cu:418-443 with the right-hand side of
  * 473-504):  y_0 = d_0, y_k = d_k - L_k y_{k-1};  x_{nz-1} = y_{nz-1}/u_{nz-1},
- * x_k = (y_k - c_k x_{k+1})/u_k.
+ * x_k = (y_k - c_k x_{k+1})/u_k, in arithmetic type TA.
  * Wave w owns levels [w LC, w LC + LC) of all 64 systems and keeps them in
  * registers.  Both recurrences are affine in the value entering the chunk, so
  * each wave runs them from a zero entry value while carrying the coefficient
  * of the entry value (products of -L_k, resp. of -c_k/u_k); the waves then
  * combine their chunk ends through LDS in order and add the correction. */
-template <int LC>
-__global__ __launch_bounds__(64 * WF_THOMAS_MAX_CHUNKS) void k_thomas(WfSpectral sp) {
-    if (sp.ctl->done) return;
-    __shared__ double2 sv[WF_THOMAS_MAX_CHUNKS][64];
-    __shared__ double sg[WF_THOMAS_MAX_CHUNKS][64];
+template <int LC, typename TA>
+__device__ __forceinline__ void wf_thomas_solve(const WfSpectral &sp, wf_cx<TA> (*sv)[64], TA (*sg)[64]) {
     const int lane = threadIdx.x & 63;
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int C = blockDim.x >> 6;
@@ -450,48 +475,48 @@ __global__ __launch_bounds__(64 * WF_THOMAS_MAX_CHUNKS) void k_thomas(WfSpectral
     const long slab = (long)sp.hx * sp.ldj;
     const long o = (long)r * sp.ldj + (ok ? j : 0);
     const int k0 = w * LC;
-    const double K2 = ok ? sp.kx2[r] + sp.ky2[j] : 0.0;
-    const double c0 = sp.c0a - 0.5 * sp.dz0 * K2 - sp.c0b;
-    double2 y[LC];
-    double g[LC], ru[LC];
-    double u = (w > 0 && ok) ? sp.UB[o + w * slab] : sp.b0; /* pivot entering the chunk */
+    const TA K2 = ok ? wf_tab<TA>(sp.kx2, sp.kx2d, r) + wf_tab<TA>(sp.ky2, sp.ky2d, j) : (TA)0.0;
+    const TA c0 = (TA)sp.c0a - (TA)0.5 * (TA)sp.dz0 * K2 - (TA)sp.c0b;
+    wf_cx<TA> y[LC];
+    TA g[LC], ru[LC];
+    TA u = (w > 0 && ok) ? (TA)sp.UB[o + w * slab] : (TA)sp.b0; /* pivot entering the chunk */
 #pragma unroll
     for (int t = 0; t < LC; t++) {
         const int k = k0 + t;
-        y[t] = make_double2(0.0, 0.0);
-        g[t] = 0.0;
-        ru[t] = 0.0;
+        y[t] = wf_cx<TA>{(TA)0.0, (TA)0.0};
+        g[t] = (TA)0.0;
+        ru[t] = (TA)0.0;
         if (ok && k < nz) {
             if (k == 0) {
-                double2 f1 = sp.F[o + slab];
-                y[t] = make_double2(0.5 * sp.dz0 * f1.x, 0.5 * sp.dz0 * f1.y);
-                u = sp.b0;
+                const wf_cx<TA> f1 = wf_ldF<TA>(sp.F, o + slab);
+                y[t] = wf_cx<TA>{(TA)0.5 * (TA)sp.dz0 * f1.x, (TA)0.5 * (TA)sp.dz0 * f1.y};
+                u = (TA)sp.b0;
             } else {
-                double2 d = sp.F[o + k * slab];
+                wf_cx<TA> d = wf_ldF<TA>(sp.F, o + k * slab);
                 if (k == nz - 1) {
-                    double2 pt = sp.Ptop[o];
-                    d.x = d.x - pt.x / sp.top_div;
-                    d.y = d.y - pt.y / sp.top_div;
+                    const wf_cx<TA> pt = wf_ldF<TA>(sp.Ptop, o);
+                    d.x = d.x - pt.x / (TA)sp.top_div;
+                    d.y = d.y - pt.y / (TA)sp.top_div;
                 }
                 y[t] = d;
-                double l;
-                u = wf_pivot_next(sp, u, k, K2, c0, &l);
+                TA l;
+                u = wf_pivot_next<TA>(sp, u, k, K2, c0, &l);
                 g[t] = l;
             }
-            ru[t] = 1.0 / u;
+            ru[t] = (TA)1.0 / u;
         }
     }
     /* forward elimination, entry value 0, coefficient G of the entry value */
-    double2 acc = make_double2(0.0, 0.0);
-    double G = 1.0;
+    wf_cx<TA> acc = {(TA)0.0, (TA)0.0};
+    TA G = (TA)1.0;
 #pragma unroll
     for (int t = 0; t < LC; t++) {
         const int k = k0 + t;
         if (k == 0) {
             acc = y[0];
-            G = 0.0;
+            G = (TA)0.0;
         } else {
-            const double m = g[t];
+            const TA m = g[t];
             acc.x = y[t].x - m * acc.x;
             acc.y = y[t].y - m * acc.y;
             G = -m * G;
@@ -502,10 +527,10 @@ __global__ __launch_bounds__(64 * WF_THOMAS_MAX_CHUNKS) void k_thomas(WfSpectral
     sv[w][lane] = acc;
     sg[w][lane] = G;
     __syncthreads();
-    double2 Y = make_double2(0.0, 0.0);
+    wf_cx<TA> Y = {(TA)0.0, (TA)0.0};
     for (int q = 0; q < w; q++) {
-        const double2 e = sv[q][lane];
-        const double gq = sg[q][lane];
+        const wf_cx<TA> e = sv[q][lane];
+        const TA gq = sg[q][lane];
         Y.x = e.x + gq * Y.x;
         Y.y = e.y + gq * Y.y;
     }
@@ -516,8 +541,8 @@ __global__ __launch_bounds__(64 * WF_THOMAS_MAX_CHUNKS) void k_thomas(WfSpectral
     }
     __syncthreads();
     /* back substitution, entry value x_{k0+LC} = 0, coefficient H */
-    double2 xa = make_double2(0.0, 0.0);
-    double H = 1.0;
+    wf_cx<TA> xa = {(TA)0.0, (TA)0.0};
+    TA H = (TA)1.0;
 #pragma unroll
     for (int t = LC - 1; t >= 0; t--) {
         const int k = k0 + t;
@@ -525,9 +550,9 @@ __global__ __launch_bounds__(64 * WF_THOMAS_MAX_CHUNKS) void k_thomas(WfSpectral
         if (k == nz - 1) {
             xa.x = y[t].x * ru[t];
             xa.y = y[t].y * ru[t];
-            H = 0.0;
+            H = (TA)0.0;
         } else {
-            const double ck = (k == 0) ? c0 : sp.tc[k];
+            const TA ck = (k == 0) ? c0 : wf_tab<TA>(sp.tc, sp.tcd, k);
             xa.x = (y[t].x - ck * xa.x) * ru[t];
             xa.y = (y[t].y - ck * xa.y) * ru[t];
             H = (-ck * ru[t]) * H;
@@ -538,10 +563,10 @@ __global__ __launch_bounds__(64 * WF_THOMAS_MAX_CHUNKS) void k_thomas(WfSpectral
     sv[w][lane] = xa;
     sg[w][lane] = H;
     __syncthreads();
-    double2 X = make_double2(0.0, 0.0);
+    wf_cx<TA> X = {(TA)0.0, (TA)0.0};
     for (int q = C - 1; q > w; q--) {
-        const double2 e = sv[q][lane];
-        const double hq = sg[q][lane];
+        const wf_cx<TA> e = sv[q][lane];
+        const TA hq = sg[q][lane];
         X.x = e.x + hq * X.x;
         X.y = e.y + hq * X.y;
     }
@@ -549,10 +574,31 @@ __global__ __launch_bounds__(64 * WF_THOMAS_MAX_CHUNKS) void k_thomas(WfSpectral
 #pragma unroll
     for (int t = 0; t < LC; t++) {
         const int k = k0 + t;
-        if (k < nz) sp.F[o + k * slab] = make_double2(y[t].x + g[t] * X.x, y[t].y + g[t] * X.y);
+        if (k < nz) sp.F[o + k * slab] = wf_make2((wf_real)(y[t].x + g[t] * X.x), (wf_real)(y[t].y + g[t] * X.y));
     }
 }
 
+template <int LC>
+__global__ __launch_bounds__(64 * WF_THOMAS_MAX_CHUNKS) void k_thomas(WfSpectral sp) {
+    if (sp.ctl->done) return;
+    __shared__ wf_cx<double> sv[WF_THOMAS_MAX_CHUNKS][64];
+    __shared__ double sg[WF_THOMAS_MAX_CHUNKS][64];
+#if WF_MIXED
+    /* a workgroup holding one of the lowest wavenumbers (kx index below
+     * low_kx, ky index within low_ky of 0 or of ny) solves in FP64: those
+     * systems are singular or nearly so under the Neumann z conditions */
+    const int jlo = blockIdx.x * 64, jhi = (jlo + 63 < sp.ny - 1) ? jlo + 63 : sp.ny - 1;
+    const bool low = ((int)blockIdx.y < sp.low_kx) && (jlo < sp.low_ky || sp.ny - jhi < sp.low_ky);
+    if (low)
+        wf_thomas_solve<LC, double>(sp, sv, sg);
+    else
+        wf_thomas_solve<LC, float>(sp, (wf_cx<float>(*)[64])sv, (float(*)[64])sg);
+#else
+    wf_thomas_solve<LC, double>(sp, sv, sg);
+#endif
+}
+
+#if !WF_MIXED /* FP64 only: it has no FP64 path for the lowest wavenumbers */
 /* P3 for nz <= 2 WF_THOMAS_MAX_CHUNKS LC (LC = 4: the 128-level grids): the
  * same chunked solve with each wave taking 32 systems and its two half-waves
  * two consecutive chunks, so a 16-wave workgroup covers 32 chunks with a
@@ -566,13 +612,13 @@ __global__ __launch_bounds__(64 * WF_THOMAS_MAX_CHUNKS) __attribute__((amdgpu_wa
     WfSpectral sp) { /* 8 waves per SIMD: <= 64 VGPRs, which this fits without spills */
     if (sp.ctl->done) return;
     constexpr int CH = 2 * WF_THOMAS_MAX_CHUNKS;
-    __shared__ double2 sv[CH][32];
-    __shared__ double sg[CH][32];
-    __shared__ double s_ta[CH * LC], s_tc[CH * LC], s_beta[CH * LC];
+    __shared__ wf_real2 sv[CH][32];
+    __shared__ wf_real sg[CH][32];
+    __shared__ wf_real s_ta[CH * LC], s_tc[CH * LC], s_beta[CH * LC];
     const int nz = sp.nz;
     for (int t = threadIdx.x; t < nz; t += blockDim.x) {
-        s_ta[t] = (t < nz - 1) ? sp.ta[t] : 0.0;
-        s_tc[t] = (t < nz - 1) ? sp.tc[t] : 0.0;
+        s_ta[t] = (t < nz - 1) ? sp.ta[t] : WR(0.0);
+        s_tc[t] = (t < nz - 1) ? sp.tc[t] : WR(0.0);
         s_beta[t] = sp.beta[t];
     }
     const int lane = threadIdx.x & 63, ls = lane & 31;
@@ -584,54 +630,54 @@ __global__ __launch_bounds__(64 * WF_THOMAS_MAX_CHUNKS) __attribute__((amdgpu_wa
     const long slab = (long)sp.hx * sp.ldj;
     const long o = (long)r * sp.ldj + (ok ? j : 0);
     const int k0 = ch * LC;
-    const double K2 = ok ? sp.kx2[r] + sp.ky2[j] : 0.0;
-    const double c0 = sp.c0a - 0.5 * sp.dz0 * K2 - sp.c0b;
+    const wf_real K2 = ok ? sp.kx2[r] + sp.ky2[j] : WR(0.0);
+    const wf_real c0 = sp.c0a - WR(0.5) * sp.dz0 * K2 - sp.c0b;
     __syncthreads();
-    double2 y[LC];
-    double g[LC], ru[LC];
+    wf_real2 y[LC];
+    wf_real g[LC], ru[LC];
     /* pivot entering the chunk (a chunk wholly above the top, possible when the
      * chunk count is odd, has no table entry and no levels) */
-    double u = (ch > 0 && ok && k0 < nz) ? sp.UB[o + ch * slab] : sp.b0;
+    wf_real u = (ch > 0 && ok && k0 < nz) ? sp.UB[o + ch * slab] : sp.b0;
 #pragma unroll
     for (int t = 0; t < LC; t++) {
         const int k = k0 + t;
-        y[t] = make_double2(0.0, 0.0);
-        g[t] = 0.0;
-        ru[t] = 0.0;
+        y[t] = wf_make2(WR(0.0), WR(0.0));
+        g[t] = WR(0.0);
+        ru[t] = WR(0.0);
         if (ok && k < nz) {
             if (k == 0) {
-                double2 f1 = sp.F[o + slab];
-                y[t] = make_double2(0.5 * sp.dz0 * f1.x, 0.5 * sp.dz0 * f1.y);
+                wf_real2 f1 = sp.F[o + slab];
+                y[t] = wf_make2(WR(0.5) * sp.dz0 * f1.x, WR(0.5) * sp.dz0 * f1.y);
                 u = sp.b0;
             } else {
-                double2 d = sp.F[o + k * slab];
+                wf_real2 d = sp.F[o + k * slab];
                 if (k == nz - 1) {
-                    double2 pt = sp.Ptop[o];
+                    wf_real2 pt = sp.Ptop[o];
                     d.x = d.x - pt.x / sp.top_div;
                     d.y = d.y - pt.y / sp.top_div;
                 }
                 y[t] = d;
                 /* wf_pivot_next with the staged tables */
-                const double ck1 = (k == 1) ? c0 : s_tc[k - 1];
-                const double bk = (k == nz - 1) ? -(K2 + sp.beta_top) : -(K2 + s_beta[k]);
-                const double l = s_ta[k - 1] / u;
+                const wf_real ck1 = (k == 1) ? c0 : s_tc[k - 1];
+                const wf_real bk = (k == nz - 1) ? -(K2 + sp.beta_top) : -(K2 + s_beta[k]);
+                const wf_real l = s_ta[k - 1] / u;
                 u = bk - l * ck1;
                 g[t] = l;
             }
-            ru[t] = 1.0 / u;
+            ru[t] = WR(1.0) / u;
         }
     }
     /* forward elimination, entry value 0, coefficient G of the entry value */
-    double2 acc = make_double2(0.0, 0.0);
-    double G = 1.0;
+    wf_real2 acc = wf_make2(WR(0.0), WR(0.0));
+    wf_real G = WR(1.0);
 #pragma unroll
     for (int t = 0; t < LC; t++) {
         const int k = k0 + t;
         if (k == 0) {
             acc = y[0];
-            G = 0.0;
+            G = WR(0.0);
         } else {
-            const double m = g[t];
+            const wf_real m = g[t];
             acc.x = y[t].x - m * acc.x;
             acc.y = y[t].y - m * acc.y;
             G = -m * G;
@@ -642,10 +688,10 @@ __global__ __launch_bounds__(64 * WF_THOMAS_MAX_CHUNKS) __attribute__((amdgpu_wa
     sv[ch][ls] = acc;
     sg[ch][ls] = G;
     __syncthreads();
-    double2 Y = make_double2(0.0, 0.0);
+    wf_real2 Y = wf_make2(WR(0.0), WR(0.0));
     for (int q = 0; q < ch; q++) {
-        const double2 e = sv[q][ls];
-        const double gq = sg[q][ls];
+        const wf_real2 e = sv[q][ls];
+        const wf_real gq = sg[q][ls];
         Y.x = e.x + gq * Y.x;
         Y.y = e.y + gq * Y.y;
     }
@@ -656,8 +702,8 @@ __global__ __launch_bounds__(64 * WF_THOMAS_MAX_CHUNKS) __attribute__((amdgpu_wa
     }
     __syncthreads();
     /* back substitution, entry value x_{k0+LC} = 0, coefficient H */
-    double2 xa = make_double2(0.0, 0.0);
-    double H = 1.0;
+    wf_real2 xa = wf_make2(WR(0.0), WR(0.0));
+    wf_real H = WR(1.0);
 #pragma unroll
     for (int t = LC - 1; t >= 0; t--) {
         const int k = k0 + t;
@@ -665,9 +711,9 @@ __global__ __launch_bounds__(64 * WF_THOMAS_MAX_CHUNKS) __attribute__((amdgpu_wa
         if (k == nz - 1) {
             xa.x = y[t].x * ru[t];
             xa.y = y[t].y * ru[t];
-            H = 0.0;
+            H = WR(0.0);
         } else {
-            const double ck = (k == 0) ? c0 : s_tc[k];
+            const wf_real ck = (k == 0) ? c0 : s_tc[k];
             xa.x = (y[t].x - ck * xa.x) * ru[t];
             xa.y = (y[t].y - ck * xa.y) * ru[t];
             H = (-ck * ru[t]) * H;
@@ -678,10 +724,10 @@ __global__ __launch_bounds__(64 * WF_THOMAS_MAX_CHUNKS) __attribute__((amdgpu_wa
     sv[ch][ls] = xa;
     sg[ch][ls] = H;
     __syncthreads();
-    double2 X = make_double2(0.0, 0.0);
+    wf_real2 X = wf_make2(WR(0.0), WR(0.0));
     for (int q = C - 1; q > ch; q--) {
-        const double2 e = sv[q][ls];
-        const double hq = sg[q][ls];
+        const wf_real2 e = sv[q][ls];
+        const wf_real hq = sg[q][ls];
         X.x = e.x + hq * X.x;
         X.y = e.y + hq * X.y;
     }
@@ -689,9 +735,10 @@ __global__ __launch_bounds__(64 * WF_THOMAS_MAX_CHUNKS) __attribute__((amdgpu_wa
 #pragma unroll
     for (int t = 0; t < LC; t++) {
         const int k = k0 + t;
-        if (k < nz) sp.F[o + k * slab] = make_double2(y[t].x + g[t] * X.x, y[t].y + g[t] * X.y);
+        if (k < nz) sp.F[o + k * slab] = wf_make2(y[t].x + g[t] * X.x, y[t].y + g[t] * X.y);
     }
 }
+#endif
 
 /* P5: inverse x-transform of one row-pair group of slab k (0..nz-1):
  * p = Re(.)/(nx ny) (multiplied by the reciprocal) on unique nodes, duplicate
@@ -702,32 +749,32 @@ __global__ __launch_bounds__(64 * WF_THOMAS_MAX_CHUNKS) __attribute__((amdgpu_wa
 
 /* new value pv of node (i, j, k) and of its periodic duplicates, with the
  * running max |p_new - p_old| */
-__device__ __forceinline__ void wf_put_p_old(const WfConst &c, double *p, int i, int j, int k, double pv, double pold,
+__device__ __forceinline__ void wf_put_p_old(const WfConst &c, wf_real *p, int i, int j, int k, wf_real pv, wf_real pold,
                                              double &dmax) {
     const long q = wf_q(c, i, j, k);
-    const double d = fabs(pv - pold);
+    const double d = (double)fabs(pv - pold);
     dmax = (d > dmax) ? d : dmax;
     p[q] = pv;
     if (i == 0) { /* duplicate column i = Nx-1 */
         const long qd = wf_q(c, c.Nx - 1, j, k);
-        const double dd = fabs(pv - p[qd]);
+        const double dd = (double)fabs(pv - p[qd]);
         dmax = (dd > dmax) ? dd : dmax;
         p[qd] = pv;
     }
     if (j == 0) { /* duplicate row j = Ny-1 */
         const long qd = wf_q(c, i, c.Ny - 1, k);
-        const double dd = fabs(pv - p[qd]);
+        const double dd = (double)fabs(pv - p[qd]);
         dmax = (dd > dmax) ? dd : dmax;
         p[qd] = pv;
         if (i == 0) {
             const long qc = wf_q(c, c.Nx - 1, c.Ny - 1, k);
-            const double dc = fabs(pv - p[qc]);
+            const double dc = (double)fabs(pv - p[qc]);
             dmax = (dc > dmax) ? dc : dmax;
             p[qc] = pv;
         }
     }
 }
-__device__ __forceinline__ void wf_put_p(const WfConst &c, double *p, int i, int j, int k, double pv, double &dmax) {
+__device__ __forceinline__ void wf_put_p(const WfConst &c, wf_real *p, int i, int j, int k, wf_real pv, double &dmax) {
     wf_put_p_old(c, p, i, j, k, pv, p[wf_q(c, i, j, k)], dmax);
 }
 
@@ -739,20 +786,20 @@ __device__ __forceinline__ void wf_put_p(const WfConst &c, double *p, int i, int
  * the group that holds j = 0).  The set of (node, new value) pairs, and so
  * the norm, is the one of wf_put_p. */
 template <int NX>
-__device__ __forceinline__ void wf_inv_fx_planned(const WfConst &c, const WfSpectral &sp, double *p, wfc *smem,
+__device__ __forceinline__ void wf_inv_fx_planned(const WfConst &c, const WfSpectral &sp, wf_real *p, wfc *smem,
                                                   int g, int k, double &dmax) {
     constexpr int PAIRS = wf_pairs_for(NX);
     constexpr int PER = (PAIRS * NX + WF_NT - 1) / WF_NT;
     constexpr int PERY = (NX + 1 + WF_NT - 1) / WF_NT; /* row j = Ny-1: i = 0 .. Nx-1 */
     const int ny = sp.ny, hx = sp.hx, ldj = sp.ldj;
     const int j0 = 2 * g * PAIRS;
-    const double2 *src = sp.F + (long)k * hx * ldj;
-    const double inv_scale = 1.0 / ((double)NX * (double)ny);
+    const wf_real2 *src = sp.F + (long)k * hx * ldj;
+    const wf_real inv_scale = WR(1.0) / ((wf_real)NX * (wf_real)ny);
     const int tid = threadIdx.x;
     wfc *tw = smem, *buf = smem + NX;
     /* loads: half spectra (thread t -> pair t % PAIRS, bin t / PAIRS, so
      * consecutive threads read consecutive ky of one kx column) */
-    double2 xa[PER], xb[PER];
+    wf_real2 xa[PER], xb[PER];
 #pragma unroll
     for (int pi = 0; pi < PER; pi++) {
         const int t = tid + pi * WF_NT;
@@ -764,7 +811,7 @@ __device__ __forceinline__ void wf_inv_fx_planned(const WfConst &c, const WfSpec
         xb[pi] = src[(long)rr * ldj + ((jb < ny) ? jb : ny - 1)];
     }
     /* old p of the unique nodes (thread t -> pair t / NX, i = t % NX) */
-    double pa[PER], pb[PER];
+    wf_real pa[PER], pb[PER];
 #pragma unroll
     for (int pi = 0; pi < PER; pi++) {
         const int t = tid + pi * WF_NT;
@@ -777,8 +824,8 @@ __device__ __forceinline__ void wf_inv_fx_planned(const WfConst &c, const WfSpec
     /* old p of the duplicates: column i = Nx-1 of the group's rows, and
      * row j = Ny-1 (i = 0 .. Nx-1) when the group holds j = 0 */
     const int jr = j0 + ((tid < 2 * PAIRS) ? tid : 0);
-    const double pdx = p[wf_q(c, c.Nx - 1, (jr < ny) ? jr : ny - 1, k)];
-    double pdy[PERY];
+    const wf_real pdx = p[wf_q(c, c.Nx - 1, (jr < ny) ? jr : ny - 1, k)];
+    wf_real pdy[PERY];
     if (g == 0) {
 #pragma unroll
         for (int pi = 0; pi < PERY; pi++) {
@@ -787,7 +834,7 @@ __device__ __forceinline__ void wf_inv_fx_planned(const WfConst &c, const WfSpec
         }
     }
     for (int t = tid; t < NX; t += WF_NT) {
-        const double2 w = sp.twx[t];
+        const wf_real2 w = sp.twx[t];
         tw[t].re = w.x;
         tw[t].im = -w.y;
     }
@@ -797,7 +844,7 @@ __device__ __forceinline__ void wf_inv_fx_planned(const WfConst &c, const WfSpec
         const int s = t % PAIRS, r = t / PAIRS;
         const int ja = j0 + 2 * s, jb = ja + 1;
         if (t < PAIRS * NX) {
-            wfc A = {0.0, 0.0}, B = {0.0, 0.0};
+            wfc A = {WR(0.0), WR(0.0)}, B = {WR(0.0), WR(0.0)};
             if (ja < ny) {
                 A.re = xa[pi].x;
                 A.im = xa[pi].y;
@@ -820,14 +867,14 @@ __device__ __forceinline__ void wf_inv_fx_planned(const WfConst &c, const WfSpec
         if (t < PAIRS * NX) {
             const wfc v = buf[s * NX + i];
             if (ja < ny) {
-                const double pv = v.re * inv_scale;
-                const double d = fabs(pv - pa[pi]);
+                const wf_real pv = v.re * inv_scale;
+                const double d = (double)fabs(pv - pa[pi]);
                 dmax = (d > dmax) ? d : dmax;
                 p[wf_q(c, i, ja, k)] = pv;
             }
             if (jb < ny) {
-                const double pv = v.im * inv_scale;
-                const double d = fabs(pv - pb[pi]);
+                const wf_real pv = v.im * inv_scale;
+                const double d = (double)fabs(pv - pb[pi]);
                 dmax = (d > dmax) ? d : dmax;
                 p[wf_q(c, i, jb, k)] = pv;
             }
@@ -836,8 +883,8 @@ __device__ __forceinline__ void wf_inv_fx_planned(const WfConst &c, const WfSpec
     /* duplicate column i = Nx-1 of rows j0 .. j0 + 2 PAIRS - 1 (value of i = 0) */
     if (tid < 2 * PAIRS && jr < ny) {
         const wfc v = buf[(tid >> 1) * NX];
-        const double pv = ((tid & 1) ? v.im : v.re) * inv_scale;
-        const double d = fabs(pv - pdx);
+        const wf_real pv = ((tid & 1) ? v.im : v.re) * inv_scale;
+        const double d = (double)fabs(pv - pdx);
         dmax = (d > dmax) ? d : dmax;
         p[wf_q(c, c.Nx - 1, jr, k)] = pv;
     }
@@ -847,8 +894,8 @@ __device__ __forceinline__ void wf_inv_fx_planned(const WfConst &c, const WfSpec
         for (int pi = 0; pi < PERY; pi++) {
             const int i = tid + pi * WF_NT;
             if (i <= NX) {
-                const double pv = buf[(i < NX) ? i : 0].re * inv_scale;
-                const double d = fabs(pv - pdy[pi]);
+                const wf_real pv = buf[(i < NX) ? i : 0].re * inv_scale;
+                const double d = (double)fabs(pv - pdy[pi]);
                 dmax = (d > dmax) ? d : dmax;
                 p[wf_q(c, i, c.Ny - 1, k)] = pv;
             }
@@ -857,7 +904,7 @@ __device__ __forceinline__ void wf_inv_fx_planned(const WfConst &c, const WfSpec
 }
 
 template <int NX>
-__global__ __launch_bounds__(WF_NT) void k_inv_fx(WfConst c, WfSpectral sp, double *p) {
+__global__ __launch_bounds__(WF_NT) void k_inv_fx(WfConst c, WfSpectral sp, wf_real *p) {
     if (sp.ctl->done) return;
     extern __shared__ wfc smem[];
     __shared__ int rad[WF_MAX_RADICES];
@@ -876,8 +923,8 @@ __global__ __launch_bounds__(WF_NT) void k_inv_fx(WfConst c, WfSpectral sp, doub
                 qd = wf_q(c, t - c.Ny, c.Ny - 1, kt);
                 qs = wf_q(c, t - c.Ny, 0, kt);
             }
-            double v = p[qs];
-            double d = fabs(v - p[qd]);
+            wf_real v = p[qs];
+            double d = (double)fabs(v - p[qd]);
             dmax = (d > dmax) ? d : dmax;
             p[qd] = v;
         }
@@ -890,12 +937,12 @@ __global__ __launch_bounds__(WF_NT) void k_inv_fx(WfConst c, WfSpectral sp, doub
         wf_inv_fx_planned<NX>(c, sp, p, smem, g, k, dmax);
     } else {
         const int j0 = 2 * g * pairs;
-        const double2 *src = sp.F + (long)k * hx * ldj;
-        const double inv_scale = 1.0 / ((double)nx * (double)ny);
+        const wf_real2 *src = sp.F + (long)k * hx * ldj;
+        const wf_real inv_scale = WR(1.0) / ((wf_real)nx * (wf_real)ny);
         const int Nx = c.Nx;
         wfc *tw = smem, *bufa = smem + nx, *bufb = bufa + pairs * nx;
         for (int t = threadIdx.x; t < nx; t += blockDim.x) {
-            double2 w = sp.twx[t];
+            wf_real2 w = sp.twx[t];
             tw[t].re = w.x;
             tw[t].im = -w.y;
         }
@@ -903,15 +950,15 @@ __global__ __launch_bounds__(WF_NT) void k_inv_fx(WfConst c, WfSpectral sp, doub
         for (int t = threadIdx.x; t < pairs * nx; t += blockDim.x) {
             const int s = t % pairs, r = t / pairs;
             const int ja = j0 + 2 * s, jb = ja + 1;
-            wfc A = {0.0, 0.0}, B = {0.0, 0.0};
+            wfc A = {WR(0.0), WR(0.0)}, B = {WR(0.0), WR(0.0)};
             const int conj = r >= hx;
             const int rr = conj ? nx - r : r;
             if (ja < ny) {
-                double2 x = src[(long)rr * ldj + ja];
+                wf_real2 x = src[(long)rr * ldj + ja];
                 A.re = x.x;
                 A.im = x.y;
                 if (jb < ny) {
-                    double2 y = src[(long)rr * ldj + jb];
+                    wf_real2 y = src[(long)rr * ldj + jb];
                     B.re = y.x;
                     B.im = y.y;
                 }
@@ -980,7 +1027,7 @@ static size_t wf_fy_lds(int n, int nseq) {
         }                                                                                               \
     } while (0)
 
-hipError_t wf_launch_density(const WfConst &c, const double *T, double *rho, hipStream_t s) {
+hipError_t wf_launch_density(const WfConst &c, const wf_real *T, wf_real *rho, hipStream_t s) {
     hipLaunchKernelGGL(k_density, dim3((unsigned)((c.N + 255) / 256)), dim3(256), 0, s, c, T, rho);
     return hipGetLastError();
 }
@@ -991,7 +1038,7 @@ hipError_t wf_launch_iter_reset(WfIterCtl *ctl, hipStream_t s) {
 }
 
 hipError_t wf_launch_pressure_rhs_fx(const WfConst &c, const WfSpectral &sp, const WfState &Ys, const WfState &Yn,
-                                     const double *p, int mode, hipStream_t s) {
+                                     const wf_real *p, int mode, hipStream_t s) {
     const int nslabs = (mode == WF_F_TOP) ? 1 : sp.nz - 1;
     dim3 grd((unsigned)(sp.ngroups * nslabs));
     size_t lds = wf_fx_lds(sp.nx, sp.fx_pairs);
@@ -1004,14 +1051,14 @@ hipError_t wf_launch_pressure_rhs_fx(const WfConst &c, const WfSpectral &sp, con
     return hipGetLastError();
 }
 
-hipError_t wf_launch_fy(const WfSpectral &sp, double2 *buf, long slab_stride, int slab0, int nslabs, int sign,
+hipError_t wf_launch_fy(const WfSpectral &sp, wf_real2 *buf, long slab_stride, int slab0, int nslabs, int sign,
                         int guarded, hipStream_t s) {
     if (nslabs <= 0) return hipSuccess;
     dim3 grd((sp.hx + sp.fy_cols - 1) / sp.fy_cols, nslabs, 1);
     size_t lds = wf_fy_lds(sp.ny, sp.fy_cols);
     const int nt = wf_f2_n1(sp.ny) ? wf_f2_threads(sp.ny) : WF_NT;
     WF_DISPATCH_F2(sp.ny, k_fy, grd, dim3(nt), lds, s, sp, buf, slab_stride, slab0, sign, guarded,
-                   (const double2 *)sp.twy);
+                   (const wf_real2 *)sp.twy);
     return hipGetLastError();
 }
 
@@ -1025,9 +1072,13 @@ hipError_t wf_launch_thomas(const WfSpectral &sp, hipStream_t s) {
     dim3 grd((sp.ny + 63) / 64, sp.hx, 1);
     const int chunks = (sp.nz + sp.thomas_lc - 1) / sp.thomas_lc;
     if (sp.thomas_lc == 4) {
+#if WF_MIXED
+        return hipErrorInvalidValue;
+#else
         if (chunks > 2 * WF_THOMAS_MAX_CHUNKS) return hipErrorInvalidValue;
         hipLaunchKernelGGL(k_thomas_h<4>, dim3((sp.ny + 31) / 32, sp.hx, 1), dim3(64 * ((chunks + 1) / 2)), 0, s, sp);
         return hipGetLastError();
+#endif
     }
     if (chunks > WF_THOMAS_MAX_CHUNKS) return hipErrorInvalidValue;
     if (sp.thomas_lc == 8)
@@ -1044,9 +1095,11 @@ hipError_t wf_launch_iter_check(const WfSpectral &sp, double tol, int m, int max
     return hipGetLastError();
 }
 
-hipError_t wf_launch_inverse_fx(const WfConst &c, const WfSpectral &sp, double *p, hipStream_t s) {
+hipError_t wf_launch_inverse_fx(const WfConst &c, const WfSpectral &sp, wf_real *p, hipStream_t s) {
     dim3 grd((unsigned)sp.npmax);
     size_t lds = wf_fx_lds(sp.nx, sp.fx_pairs);
     WF_DISPATCH_N(sp.nx, k_inv_fx, grd, dim3(WF_NT), lds, s, c, sp, p);
     return hipGetLastError();
 }
+
+}  // namespace WF_NS

@@ -1,14 +1,17 @@
 /*
- * wf_solver.hip -- host side of the time step: device workspace, the step
- * sequence of src/cu/solver.cu:178-265 and the wf_* entry points.
+ * Solver.hip -- host side of the time step, in one precision: device
+ * workspace, the step sequence of src/cu/solver.cu:178-265 and the pieces the
+ * C-ABI exposes (wf_api.hip dispatches to the wf64 or the wf32 compilation).
  *
  * Device buffers (internal layout, see wf_common.h):
- *   state[4]  four state vectors of S doubles: y_n, two stage inputs, and the
- *             running RK combination; roles rotate between steps
- *   derived   8 N + Nx Ny: per-node fields the SGS divergence reads at neighbours
+ *   state[4]  four state vectors: u, v, w, T in wf_real and Y in FP64
+ *             (max(Nz_Y_max, 1) planes); y_n, two stage inputs and the running
+ *             RK combination; roles rotate between steps
+ *   derived   8 N + 21 Nx Ny: per-node fields the SGS divergence reads at
+ *             neighbours, u_tau and the kept raw RHS of the column passes
  *   p, A      pressure and the p-independent part of the Poisson RHS
- *   F, Ptop   x-half-spectrum working buffer [Nz-1][Ny-1][(Nx-1)/2+1] and the
- *             spectrum of the top pressure plane; U the Thomas pivots
+ *   F, Ptop   x-half-spectrum working buffer [Nz-1][(Nx-1)/2+1][ldj] and the
+ *             spectrum of the top pressure plane; UB the Thomas pivots (FP64)
  */
 #include <math.h>
 #include <stdint.h>
@@ -22,24 +25,11 @@
 
 #include "../../../include/wildfires.h"
 #include "wf_common.h"
+#include "wf_core.h"
 #include "wf_fft.h"
 #include "wf_kernels.h"
 
-#define WF_VERSION "wildfires_amd 0.1 (gfx950, FP64)"
 #define WF_MAX_EVENTS 512
-
-static thread_local char wf_err[512] = "";
-
-static int wf_fail(int code, const char *fmt, ...) {
-    va_list ap;
-    va_start(ap, fmt);
-    vsnprintf(wf_err, sizeof(wf_err), fmt, ap);
-    va_end(ap);
-    return code;
-}
-
-/* for the other translation units of the library (wf_closure_train.hip) */
-int wf_set_error(int code, const char *msg) { return wf_fail(code, "%s", msg); }
 
 #define WF_HIP(call)                                                                              \
     do {                                                                                          \
@@ -48,28 +38,32 @@ int wf_set_error(int code, const char *msg) { return wf_fail(code, "%s", msg); }
             return wf_fail(WF_E_DEVICE, "%s:%d %s: %s", __FILE__, __LINE__, #call, hipGetErrorString(e_)); \
     } while (0)
 
-extern "C" const char *wf_last_error(void) { return wf_err; }
-extern "C" const char *wf_version(void) { return WF_VERSION; }
+namespace WF_NS {
 
-struct wf_solver {
+struct Solver {
     Parameters P;
     WfConst c;
     WfSpectral sp;
     long N, S, plane;
+    long ny_planes;  /* Y planes on the device: max(Nz_Y_max, 1) */
+    size_t sbytes;   /* bytes of one device state vector */
     hipStream_t stream;
-    double *state[4];
+    char *state[4];
     int i_yn, i_x0, i_x1, i_acc;
-    double *derived;
-    double *p, *A, *UB, *pmax;
-    double2 *F, *Ptop;
-    double *z, *cd2, *delta_k, *zc, *topo, *kx2, *ky2, *ta, *tc, *beta, *x, *y;
+    wf_real *derived;
+    wf_real *p, *A;
+    double *UB, *pmax;
+    wf_real2 *F, *Ptop;
+    wf_real *z, *cd2, *delta_k, *zc, *topo, *kx2, *ky2, *ta, *tc, *beta, *x, *y;
+    double *kx2d, *ky2d, *tad, *tcd, *betad; /* FP64 copies (the mixed form's low wavenumbers) */
     /* learned closure (off unless wf_solver_set_closure gave parameters) */
     float *cl_prm;
     wf_bf16x8 *cl_frag;
     int closure_on;
     int *cut, *nzy, *radx, *rady;
-    double2 *twx, *twy;
-    double *scratch, *Tsrc, *rho;
+    wf_real2 *twx, *twy;
+    double *scratch; /* S doubles: host transfers in the reference layout */
+    wf_real *Tsrc, *rho;
     int have_Tsrc;
     WfIterCtl *ctl;
     int *hflag;      /* mapped pinned host memory: [0] done, [1] solves finished */
@@ -87,19 +81,21 @@ struct wf_solver {
     long k_count[WF_K_COUNT];
 };
 
-static WfState wf_view(const wf_solver *s, double *base) {
+/* u, v, w, T (wf_real) followed by Y (FP64; 4 N wf_real is a multiple of 8 bytes) */
+static WfState wf_view(const Solver *s, char *base) {
     WfState v;
-    v.u = base;
-    v.v = base + s->N;
-    v.w = base + 2 * s->N;
-    v.T = base + 3 * s->N;
-    v.Y = base + 4 * s->N;
+    wf_real *f = (wf_real *)base;
+    v.u = f;
+    v.v = f + s->N;
+    v.w = f + 2 * s->N;
+    v.T = f + 3 * s->N;
+    v.Y = (double *)(base + sizeof(wf_real) * 4 * s->N);
     return v;
 }
 
-static WfDerived wf_derived_view(const wf_solver *s) {
+static WfDerived wf_derived_view(const Solver *s) {
     WfDerived d;
-    double *b = s->derived;
+    wf_real *b = s->derived;
     long N = s->N;
     d.mu = b;
     d.s11 = b + N;
@@ -141,9 +137,23 @@ static int wf_upload(T **dst, const T *src, size_t count) {
     return WF_OK;
 }
 
+/* an FP64 host table in wf_real on the device (rounded once, on the host) */
+static int wf_upload_real(wf_real **dst, const double *src, size_t count) {
+    std::vector<wf_real> v(src, src + count);
+    return wf_upload(dst, v.data(), count);
+}
+static int wf_upload_real2(wf_real2 **dst, const std::vector<double2> &src) {
+    std::vector<wf_real2> v(src.size());
+    for (size_t q = 0; q < src.size(); q++) {
+        v[q].x = (wf_real)src[q].x;
+        v[q].y = (wf_real)src[q].y;
+    }
+    return wf_upload(dst, v.data(), v.size());
+}
+
 /* ------------------------------------------------------------ profiling ---- */
 
-static int wf_prof_begin(wf_solver *s, int kind) {
+static int wf_prof_begin(Solver *s, int kind) {
     if (!s->profiling || s->n_ev >= WF_MAX_EVENTS) return -1;
     int slot = s->n_ev++;
     s->ev_kind[slot] = kind;
@@ -151,17 +161,17 @@ static int wf_prof_begin(wf_solver *s, int kind) {
     return slot;
 }
 
-static void wf_prof_end(wf_solver *s, int slot) {
+static void wf_prof_end(Solver *s, int slot) {
     if (slot >= 0) (void)hipEventRecord(s->ev[2 * slot + 1], s->stream);
 }
 
-static int wf_prof_collect(wf_solver *s) {
+static int wf_prof_collect(Solver *s) {
     if (!s->profiling) return WF_OK;
     WF_HIP(hipStreamSynchronize(s->stream));
     for (int q = 0; q < s->n_ev; q++) {
         float ms = 0.0f;
         WF_HIP(hipEventElapsedTime(&ms, s->ev[2 * q], s->ev[2 * q + 1]));
-        s->k_ms[s->ev_kind[q]] += ms;
+        s->k_ms[s->ev_kind[q]] += (double)ms;
         s->k_count[s->ev_kind[q]] += 1;
     }
     s->n_ev = 0;
@@ -203,7 +213,7 @@ static int wf_method(const Parameters *P) {
     return -1;
 }
 
-static int wf_create_impl(wf_solver *s, const Parameters *P) {
+static int wf_create_impl(Solver *s, const Parameters *P) {
     int rc = wf_validate(P);
     if (rc != WF_OK) return rc;
     int ndev = 0;
@@ -215,24 +225,26 @@ static int wf_create_impl(wf_solver *s, const Parameters *P) {
     s->plane = (long)Nx * Ny;
     s->N = s->plane * Nz;
     s->S = 4 * s->N + s->plane * Nzm;
+    s->ny_planes = (Nzm > 0) ? Nzm : 1;
+    s->sbytes = sizeof(wf_real) * 4 * s->N + sizeof(double) * s->plane * s->ny_planes;
     s->method = wf_method(P);
     WF_HIP(hipStreamCreate(&s->stream));
     for (int q = 0; q < 4; q++) {
-        WF_HIP(hipMalloc((void **)&s->state[q], sizeof(double) * s->S));
-        WF_HIP(hipMemset(s->state[q], 0, sizeof(double) * s->S));
+        WF_HIP(hipMalloc((void **)&s->state[q], s->sbytes));
+        WF_HIP(hipMemset(s->state[q], 0, s->sbytes));
     }
     s->i_yn = 0;
     s->i_x0 = 1;
     s->i_x1 = 2;
     s->i_acc = 3;
-    WF_HIP(hipMalloc((void **)&s->derived, sizeof(double) * (8 * s->N + 21 * s->plane)));
-    WF_HIP(hipMemset(s->derived, 0, sizeof(double) * (8 * s->N + 21 * s->plane)));
-    WF_HIP(hipMalloc((void **)&s->p, sizeof(double) * s->N));
-    WF_HIP(hipMemset(s->p, 0, sizeof(double) * s->N));
-    WF_HIP(hipMalloc((void **)&s->A, sizeof(double) * s->N));
+    WF_HIP(hipMalloc((void **)&s->derived, sizeof(wf_real) * (8 * s->N + 21 * s->plane)));
+    WF_HIP(hipMemset(s->derived, 0, sizeof(wf_real) * (8 * s->N + 21 * s->plane)));
+    WF_HIP(hipMalloc((void **)&s->p, sizeof(wf_real) * s->N));
+    WF_HIP(hipMemset(s->p, 0, sizeof(wf_real) * s->N));
+    WF_HIP(hipMalloc((void **)&s->A, sizeof(wf_real) * s->N));
     WF_HIP(hipMalloc((void **)&s->scratch, sizeof(double) * s->S));
-    WF_HIP(hipMalloc((void **)&s->Tsrc, sizeof(double) * s->N));
-    WF_HIP(hipMalloc((void **)&s->rho, sizeof(double) * s->N));
+    WF_HIP(hipMalloc((void **)&s->Tsrc, sizeof(wf_real) * s->N));
+    WF_HIP(hipMalloc((void **)&s->rho, sizeof(wf_real) * s->N));
     WF_HIP(hipMalloc((void **)&s->ctl, sizeof(WfIterCtl)));
     WF_HIP(hipMemset(s->ctl, 0, sizeof(WfIterCtl)));
     WF_HIP(hipHostMalloc((void **)&s->hflag, 2 * sizeof(int), hipHostMallocMapped));
@@ -287,13 +299,12 @@ static int wf_create_impl(wf_solver *s, const Parameters *P) {
             e[WF_Z_D2B + 3] = (2 * hm3 + 4 * hm2 + 6 * hm1) / (hm1 * (hm2 + hm1) * (hm3 + hm2 + hm1));
         }
     }
-    if ((rc = wf_upload(&s->z, z, Nz))) return rc;
-    if ((rc = wf_upload(&s->cd2, cd2.data(), Nz))) return rc;
-    if ((rc = wf_upload(&s->delta_k, delta_k.data(), Nz))) return rc;
-    if ((rc = wf_upload(&s->zc, zc.data(), zc.size()))) return rc;
-    if ((rc = wf_upload(&s->topo, P->topography, s->plane))) return rc;
-    if ((rc = wf_upload(&s->x, P->x, Nx))) return rc;
-    if ((rc = wf_upload(&s->y, P->y, Ny))) return rc;
+    if ((rc = wf_upload_real(&s->z, z, Nz))) return rc;
+    if ((rc = wf_upload_real(&s->cd2, cd2.data(), Nz))) return rc;
+    if ((rc = wf_upload_real(&s->delta_k, delta_k.data(), Nz))) return rc;
+    if ((rc = wf_upload_real(&s->zc, zc.data(), zc.size()))) return rc;
+    if ((rc = wf_upload_real(&s->x, P->x, Nx))) return rc;
+    if ((rc = wf_upload_real(&s->y, P->y, Ny))) return rc;
     /* masks: the host arrays are indexed (i, j) with j fastest; the device
      * tables are indexed col = i + Nx j */
     std::vector<int> cut(s->plane), nzy(s->plane);
@@ -304,7 +315,7 @@ static int wf_create_impl(wf_solver *s, const Parameters *P) {
             nzy[i + (long)Nx * j] = P->Nz_Y[(long)j + (long)Ny * i];
             topo[i + (long)Nx * j] = P->topography[(long)j + (long)Ny * i];
         }
-    WF_HIP(hipMemcpy(s->topo, topo.data(), sizeof(double) * s->plane, hipMemcpyHostToDevice));
+    if ((rc = wf_upload_real(&s->topo, topo.data(), s->plane))) return rc;
     if ((rc = wf_upload(&s->cut, cut.data(), s->plane))) return rc;
     if ((rc = wf_upload(&s->nzy, nzy.data(), s->plane))) return rc;
 
@@ -393,13 +404,13 @@ static int wf_create_impl(wf_solver *s, const Parameters *P) {
         double ang = 2.0 * M_PI * (double)e / (double)sp.ny;
         twy[e] = make_double2(cos(ang), -sin(ang));
     }
-    if ((rc = wf_upload(&s->twx, twx.data(), sp.nx))) return rc;
-    if ((rc = wf_upload(&s->twy, twy.data(), sp.ny))) return rc;
+    if ((rc = wf_upload_real2(&s->twx, twx))) return rc;
+    if ((rc = wf_upload_real2(&s->twy, twy))) return rc;
     std::vector<double> kx2(sp.hx), ky2(sp.ny);
     for (int r = 0; r < sp.hx; r++) kx2[r] = P->kx[r] * P->kx[r];
     for (int q = 0; q < sp.ny; q++) ky2[q] = P->ky[q] * P->ky[q];
-    if ((rc = wf_upload(&s->kx2, kx2.data(), sp.hx))) return rc;
-    if ((rc = wf_upload(&s->ky2, ky2.data(), sp.ny))) return rc;
+    if ((rc = wf_upload_real(&s->kx2, kx2.data(), sp.hx))) return rc;
+    if ((rc = wf_upload_real(&s->ky2, ky2.data(), sp.ny))) return rc;
     std::vector<double> ta(sp.nz, 0.0), tc(sp.nz, 0.0), beta(sp.nz, 0.0);
     for (int k = 0; k < Nz - 2; k++) {
         double dz_k = z[k + 1] - z[k], dz_kp1 = z[k + 2] - z[k + 1];
@@ -410,9 +421,24 @@ static int wf_create_impl(wf_solver *s, const Parameters *P) {
             beta[k] = 2 / (dz_km1 * dz_k);
         }
     }
-    if ((rc = wf_upload(&s->ta, ta.data(), sp.nz))) return rc;
-    if ((rc = wf_upload(&s->tc, tc.data(), sp.nz))) return rc;
-    if ((rc = wf_upload(&s->beta, beta.data(), sp.nz))) return rc;
+    if ((rc = wf_upload_real(&s->ta, ta.data(), sp.nz))) return rc;
+    if ((rc = wf_upload_real(&s->tc, tc.data(), sp.nz))) return rc;
+    if ((rc = wf_upload_real(&s->beta, beta.data(), sp.nz))) return rc;
+#if WF_MIXED
+    if ((rc = wf_upload(&s->kx2d, kx2.data(), sp.hx))) return rc;
+    if ((rc = wf_upload(&s->ky2d, ky2.data(), sp.ny))) return rc;
+    if ((rc = wf_upload(&s->tad, ta.data(), sp.nz))) return rc;
+    if ((rc = wf_upload(&s->tcd, tc.data(), sp.nz))) return rc;
+    if ((rc = wf_upload(&s->betad, beta.data(), sp.nz))) return rc;
+    /* the systems solved in FP64: kx index < low_kx and ky index within
+     * low_ky of 0 or ny (WF_MIXED_LOW_KX / WF_MIXED_LOW_KY override 4, 4) */
+    const char *ekx = getenv("WF_MIXED_LOW_KX"), *eky = getenv("WF_MIXED_LOW_KY");
+    sp.low_kx = ekx ? atoi(ekx) : 4;
+    sp.low_ky = eky ? atoi(eky) : 4;
+#else
+    sp.low_kx = 0;
+    sp.low_ky = 0;
+#endif
     double dz0 = z[1] - z[0], dz1 = z[2] - z[1];
     double dzNm2 = z[Nz - 2] - z[Nz - 3], dzNm1 = z[Nz - 1] - z[Nz - 2];
     sp.dz0 = dz0;
@@ -422,10 +448,10 @@ static int wf_create_impl(wf_solver *s, const Parameters *P) {
     sp.beta_top = 2 / (dzNm2 * dzNm1);
     sp.top_div = 2 / (dzNm1 * (dzNm2 + dzNm1));
     long nspec = (long)sp.nz * sp.hx * sp.ldj;
-    WF_HIP(hipMalloc((void **)&s->F, sizeof(double2) * nspec));
-    WF_HIP(hipMemset(s->F, 0, sizeof(double2) * nspec));
-    WF_HIP(hipMalloc((void **)&s->Ptop, sizeof(double2) * (long)sp.hx * sp.ldj));
-    WF_HIP(hipMemset(s->Ptop, 0, sizeof(double2) * (long)sp.hx * sp.ldj));
+    WF_HIP(hipMalloc((void **)&s->F, sizeof(wf_real2) * nspec));
+    WF_HIP(hipMemset(s->F, 0, sizeof(wf_real2) * nspec));
+    WF_HIP(hipMalloc((void **)&s->Ptop, sizeof(wf_real2) * (long)sp.hx * sp.ldj));
+    WF_HIP(hipMemset(s->Ptop, 0, sizeof(wf_real2) * (long)sp.hx * sp.ldj));
     /* 4 levels per half-wave chunk (k_thomas_h) up to 128 levels, else whole-wave chunks of 8 or 16 */
     sp.thomas_lc = (WF_THOMAS_HALF && sp.nz <= 4 * 2 * WF_THOMAS_MAX_CHUNKS) ? 4
                    : (sp.nz <= 8 * WF_THOMAS_MAX_CHUNKS)                       ? 8
@@ -445,6 +471,19 @@ static int wf_create_impl(wf_solver *s, const Parameters *P) {
     sp.ta = s->ta;
     sp.tc = s->tc;
     sp.beta = s->beta;
+#if WF_MIXED
+    sp.kx2d = s->kx2d;
+    sp.ky2d = s->ky2d;
+    sp.tad = s->tad;
+    sp.tcd = s->tcd;
+    sp.betad = s->betad;
+#else
+    sp.kx2d = s->kx2;
+    sp.ky2d = s->ky2;
+    sp.tad = s->ta;
+    sp.tcd = s->tc;
+    sp.betad = s->beta;
+#endif
     sp.twx = s->twx;
     sp.twy = s->twy;
     sp.radx = s->radx;
@@ -458,16 +497,16 @@ static int wf_create_impl(wf_solver *s, const Parameters *P) {
     return WF_OK;
 }
 
-extern "C" int wf_solver_create(const Parameters *parameters, wf_solver **out) {
+int create(const Parameters *parameters, Solver **out) {
     if (parameters == NULL || out == NULL) return wf_fail(WF_E_ARG, "null argument");
-    wf_solver *s = (wf_solver *)calloc(1, sizeof(wf_solver));
+    Solver *s = (Solver *)calloc(1, sizeof(Solver));
     if (s == NULL) return wf_fail(WF_E_ALLOC, "out of host memory");
     int rc = wf_create_impl(s, parameters);
     if (rc != WF_OK) {
         char keep[512];
-        memcpy(keep, wf_err, sizeof(keep));
-        wf_solver_destroy(s);
-        memcpy(wf_err, keep, sizeof(keep));
+        snprintf(keep, sizeof(keep), "%s", wf_error_message());
+        destroy(s);
+        wf_error_restore(keep);
         *out = NULL;
         return rc;
     }
@@ -475,14 +514,14 @@ extern "C" int wf_solver_create(const Parameters *parameters, wf_solver **out) {
     return WF_OK;
 }
 
-extern "C" int wf_solver_destroy(wf_solver *s) {
+int destroy(Solver *s) {
     if (s == NULL) return WF_OK;
     if (s->stream) (void)hipStreamSynchronize(s->stream);
     for (int q = 0; q < 4; q++) (void)hipFree(s->state[q]);
     void *bufs[] = {s->zc, s->derived, s->p, s->A, s->UB, s->pmax, s->F, s->Ptop, s->z, s->cd2, s->delta_k,
                     s->cl_prm, s->cl_frag, s->topo, s->kx2, s->ky2, s->ta,
                     s->tc, s->beta, s->x, s->y, s->cut, s->nzy, s->radx, s->rady, s->twx, s->twy, s->scratch,
-                    s->Tsrc, s->rho, s->ctl};
+                    s->Tsrc, s->rho, s->ctl, s->kx2d, s->ky2d, s->tad, s->tcd, s->betad};
     for (size_t q = 0; q < sizeof(bufs) / sizeof(bufs[0]); q++)
         if (bufs[q]) (void)hipFree(bufs[q]);
     for (int q = 0; q < 2 * WF_MAX_EVENTS; q++)
@@ -495,34 +534,38 @@ extern "C" int wf_solver_destroy(wf_solver *s) {
 
 /* ------------------------------------------------------ layout transfers ---- */
 
-/* host reference-layout state vector -> device internal-layout buffer */
-static int wf_put_state(wf_solver *s, const double *y_host, double *dst) {
+/* host reference-layout state vector (FP64) -> device internal-layout buffer */
+static int wf_put_state(Solver *s, const double *y_host, char *dst) {
     const Parameters &P = s->P;
+    const WfState d = wf_view(s, dst);
+    wf_real *f4[4] = {d.u, d.v, d.w, d.T};
     WF_HIP(hipMemcpyAsync(s->scratch, y_host, sizeof(double) * s->S, hipMemcpyHostToDevice, s->stream));
     for (int f = 0; f < 4; f++)
-        WF_HIP(wf_launch_to_internal(s->scratch + f * s->N, dst + f * s->N, P.Nx, P.Ny, P.Nz, s->stream));
-    WF_HIP(wf_launch_to_internal(s->scratch + 4 * s->N, dst + 4 * s->N, P.Nx, P.Ny, P.Nz_Y_max, s->stream));
+        WF_HIP(wf_launch_to_internal(s->scratch + f * s->N, f4[f], P.Nx, P.Ny, P.Nz, s->stream));
+    WF_HIP(wf_launch_to_internal(s->scratch + 4 * s->N, d.Y, P.Nx, P.Ny, P.Nz_Y_max, s->stream));
     return WF_OK;
 }
 
-static int wf_get_state_buf(wf_solver *s, const double *src, double *y_host) {
+static int wf_get_state_buf(Solver *s, char *src, double *y_host) {
     const Parameters &P = s->P;
+    const WfState v = wf_view(s, src);
+    const wf_real *f4[4] = {v.u, v.v, v.w, v.T};
     for (int f = 0; f < 4; f++)
-        WF_HIP(wf_launch_to_reference(src + f * s->N, s->scratch + f * s->N, P.Nx, P.Ny, P.Nz, s->stream));
-    WF_HIP(wf_launch_to_reference(src + 4 * s->N, s->scratch + 4 * s->N, P.Nx, P.Ny, P.Nz_Y_max, s->stream));
+        WF_HIP(wf_launch_to_reference(f4[f], s->scratch + f * s->N, P.Nx, P.Ny, P.Nz, s->stream));
+    WF_HIP(wf_launch_to_reference((const double *)v.Y, s->scratch + 4 * s->N, P.Nx, P.Ny, P.Nz_Y_max, s->stream));
     WF_HIP(hipMemcpyAsync(y_host, s->scratch, sizeof(double) * s->S, hipMemcpyDeviceToHost, s->stream));
     WF_HIP(hipStreamSynchronize(s->stream));
     return WF_OK;
 }
 
-static int wf_put_field(wf_solver *s, const double *f_host, double *dst) {
+static int wf_put_field(Solver *s, const double *f_host, wf_real *dst) {
     const Parameters &P = s->P;
     WF_HIP(hipMemcpyAsync(s->scratch, f_host, sizeof(double) * s->N, hipMemcpyHostToDevice, s->stream));
     WF_HIP(wf_launch_to_internal(s->scratch, dst, P.Nx, P.Ny, P.Nz, s->stream));
     return WF_OK;
 }
 
-static int wf_get_field(wf_solver *s, const double *src, double *f_host) {
+static int wf_get_field(Solver *s, const wf_real *src, double *f_host) {
     const Parameters &P = s->P;
     WF_HIP(wf_launch_to_reference(src, s->scratch, P.Nx, P.Ny, P.Nz, s->stream));
     WF_HIP(hipMemcpyAsync(f_host, s->scratch, sizeof(double) * s->N, hipMemcpyDeviceToHost, s->stream));
@@ -532,21 +575,21 @@ static int wf_get_field(wf_solver *s, const double *src, double *f_host) {
 
 /* spectrum of the top pressure plane (fixed for the whole run: the solve never
  * writes the unique nodes of k = Nz-1) */
-static int wf_top_spectrum(wf_solver *s) {
-    WfState none = {0, 0, 0, 0, 0};
+static int wf_top_spectrum(Solver *s) {
+    WfState none = {nullptr, nullptr, nullptr, nullptr, nullptr};
     WF_HIP(wf_launch_pressure_rhs_fx(s->c, s->sp, none, none, s->p, WF_F_TOP, s->stream));
     WF_HIP(wf_launch_fy(s->sp, s->Ptop, 0, 0, 1, -1, 0, s->stream));
     return WF_OK;
 }
 
-extern "C" int wf_solver_set_state(wf_solver *s, const double *y_n, const double *p) {
+int set_state(Solver *s, const double *y_n, const double *p) {
     if (s == NULL || y_n == NULL || p == NULL) return wf_fail(WF_E_ARG, "null argument");
     int rc;
     if ((rc = wf_put_state(s, y_n, s->state[s->i_yn]))) return rc;
     if ((rc = wf_put_field(s, p, s->p))) return rc;
     if (!s->have_Tsrc) {
-        WF_HIP(hipMemcpyAsync(s->Tsrc, s->state[s->i_yn] + 3 * s->N, sizeof(double) * s->N, hipMemcpyDeviceToDevice,
-                              s->stream));
+        WF_HIP(hipMemcpyAsync(s->Tsrc, wf_view(s, s->state[s->i_yn]).T, sizeof(wf_real) * s->N,
+                              hipMemcpyDeviceToDevice, s->stream));
         s->have_Tsrc = 1;
     }
     if ((rc = wf_top_spectrum(s))) return rc;
@@ -554,7 +597,7 @@ extern "C" int wf_solver_set_state(wf_solver *s, const double *y_n, const double
     return WF_OK;
 }
 
-extern "C" int wf_solver_get_state(wf_solver *s, double *y_n, double *p) {
+int get_state(Solver *s, double *y_n, double *p) {
     if (s == NULL) return wf_fail(WF_E_ARG, "null argument");
     int rc;
     if (y_n && (rc = wf_get_state_buf(s, s->state[s->i_yn], y_n))) return rc;
@@ -565,7 +608,7 @@ extern "C" int wf_solver_get_state(wf_solver *s, double *y_n, double *p) {
 /* --------------------------------------------------------------- pieces ---- */
 
 /* Phi of one stage: derived pass on X, then the stage pass in `mode`. */
-static int wf_stage(wf_solver *s, double *X, double *out, int mode, double a) {
+static int wf_stage(Solver *s, char *X, char *out, int mode, double a) {
     WfState xs = wf_view(s, X), yn = wf_view(s, s->state[s->i_yn]), acc = wf_view(s, s->state[s->i_acc]);
     WfState o = wf_view(s, out);
     WfDerived D = wf_derived_view(s);
@@ -576,9 +619,9 @@ static int wf_stage(wf_solver *s, double *X, double *out, int mode, double a) {
 }
 
 /* RK stages of src/cu/solver.cu:59-93: y_n -> y* in state[i_x1]. */
-static int wf_integrate(wf_solver *s) {
+static int wf_integrate(Solver *s) {
     double dt = s->P.dt;
-    double *yn = s->state[s->i_yn], *x0 = s->state[s->i_x0], *x1 = s->state[s->i_x1];
+    char *yn = s->state[s->i_yn], *x0 = s->state[s->i_x0], *x1 = s->state[s->i_x1];
     int rc;
     if (s->method == 2) {
         if ((rc = wf_stage(s, yn, x0, WF_MODE_FIRST, dt * 0.5))) return rc;
@@ -603,7 +646,7 @@ static int wf_integrate(wf_solver *s) {
  * host keeps at most WF_LOOKAHEAD solves queued ahead of the device and stops
  * queueing once the flag is set. */
 #define WF_LOOKAHEAD 6
-static int wf_pressure_solve(wf_solver *s, double *Ys_buf, double *Yn_buf) {
+static int wf_pressure_solve(Solver *s, char *Ys_buf, char *Yn_buf) {
     const Parameters &P = s->P;
     WfState ys = wf_view(s, Ys_buf), yn = wf_view(s, Yn_buf);
     int max_iter = (P.variable_density == 0) ? 1 : P.pressure_solver_iter;
@@ -648,7 +691,7 @@ static int wf_pressure_solve(wf_solver *s, double *Ys_buf, double *Yn_buf) {
     return WF_OK;
 }
 
-static WfSource wf_source(wf_solver *s, int n) {
+static WfSource wf_source(Solver *s, int n) {
     const Parameters &P = s->P;
     WfSource src;
     memset(&src, 0, sizeof(src));
@@ -681,7 +724,7 @@ static WfSource wf_source(wf_solver *s, int n) {
     return src;
 }
 
-extern "C" int wf_solver_step(wf_solver *s, int step0, int n_steps) {
+int step(Solver *s, int step0, int n_steps) {
     if (s == NULL) return wf_fail(WF_E_ARG, "null solver");
     if (s->method < 0) return wf_fail(WF_E_METHOD, "Time integration method not found.");
     if (step0 < 1 || step0 + n_steps - 1 > s->P.Nt)
@@ -689,7 +732,7 @@ extern "C" int wf_solver_step(wf_solver *s, int step0, int n_steps) {
     int rc;
     for (int n = step0; n < step0 + n_steps; n++) {
         if ((rc = wf_integrate(s))) return rc;
-        double *ys = s->state[s->i_x1], *yn = s->state[s->i_yn], *out = s->state[s->i_x0];
+        char *ys = s->state[s->i_x1], *yn = s->state[s->i_yn], *out = s->state[s->i_x0];
         if ((rc = wf_pressure_solve(s, ys, yn))) return rc;
         WfSource src = wf_source(s, n);
         WF_LAUNCH(s, WF_K_PROJECT,
@@ -705,21 +748,21 @@ extern "C" int wf_solver_step(wf_solver *s, int step0, int n_steps) {
     return WF_OK;
 }
 
-extern "C" int wf_solver_phi(wf_solver *s, const double *R_old, double *R_new) {
+int phi(Solver *s, const double *R_old, double *R_new) {
     if (s == NULL || R_old == NULL || R_new == NULL) return wf_fail(WF_E_ARG, "null argument");
     int rc;
-    double *x0 = s->state[s->i_x0], *x1 = s->state[s->i_x1];
+    char *x0 = s->state[s->i_x0], *x1 = s->state[s->i_x1];
     if ((rc = wf_put_state(s, R_old, x0))) return rc;
     if ((rc = wf_stage(s, x0, x1, WF_MODE_PHI, 0.0))) return rc;
     if ((rc = wf_get_state_buf(s, x1, R_new))) return rc;
     return wf_prof_collect(s);
 }
 
-extern "C" int wf_solver_pressure(wf_solver *s, const double *y_np1, const double *y_n, double *p, double *error,
+int pressure(Solver *s, const double *y_np1, const double *y_n, double *p, double *error,
                                   int *iterations) {
     if (s == NULL || y_np1 == NULL || y_n == NULL || p == NULL) return wf_fail(WF_E_ARG, "null argument");
     int rc;
-    double *x1 = s->state[s->i_x1], *yn = s->state[s->i_yn];
+    char *x1 = s->state[s->i_x1], *yn = s->state[s->i_yn];
     if ((rc = wf_put_state(s, y_np1, x1))) return rc;
     if ((rc = wf_put_state(s, y_n, yn))) return rc;
     if ((rc = wf_put_field(s, p, s->p))) return rc;
@@ -731,11 +774,11 @@ extern "C" int wf_solver_pressure(wf_solver *s, const double *y_np1, const doubl
     return wf_prof_collect(s);
 }
 
-extern "C" int wf_solver_project(wf_solver *s, double *y_np1, const double *y_n, const double *p, int n) {
+int project(Solver *s, double *y_np1, const double *y_n, const double *p, int n) {
     if (s == NULL || y_np1 == NULL || y_n == NULL || p == NULL) return wf_fail(WF_E_ARG, "null argument");
     if (n < 0 || n > s->P.Nt) return wf_fail(WF_E_ARG, "step %d outside 0..Nt", n);
     int rc;
-    double *x1 = s->state[s->i_x1], *yn = s->state[s->i_yn], *x0 = s->state[s->i_x0];
+    char *x1 = s->state[s->i_x1], *yn = s->state[s->i_yn], *x0 = s->state[s->i_x0];
     if ((rc = wf_put_state(s, y_np1, x1))) return rc;
     if ((rc = wf_put_state(s, y_n, yn))) return rc;
     if ((rc = wf_put_field(s, p, s->p))) return rc;
@@ -746,7 +789,7 @@ extern "C" int wf_solver_project(wf_solver *s, double *y_np1, const double *y_n,
     return wf_prof_collect(s);
 }
 
-extern "C" int wf_solver_stats(const wf_solver *s, long *pressure_iterations, long *steps, double *last_error) {
+int stats(const Solver *s, long *pressure_iterations, long *steps, double *last_error) {
     if (s == NULL) return wf_fail(WF_E_ARG, "null solver");
     if (pressure_iterations) *pressure_iterations = s->press_solves;
     if (steps) *steps = s->steps;
@@ -754,7 +797,7 @@ extern "C" int wf_solver_stats(const wf_solver *s, long *pressure_iterations, lo
     return WF_OK;
 }
 
-extern "C" int wf_solver_profile(wf_solver *s, int enable) {
+int profile(Solver *s, int enable) {
     if (s == NULL) return wf_fail(WF_E_ARG, "null solver");
     s->profiling = enable ? 1 : 0;
     s->n_ev = 0;
@@ -765,20 +808,15 @@ extern "C" int wf_solver_profile(wf_solver *s, int enable) {
     return WF_OK;
 }
 
-extern "C" int wf_solver_kernel_times(const wf_solver *s, int kind, double *total_ms, long *launches) {
+int kernel_times(const Solver *s, int kind, double *total_ms, long *launches) {
     if (s == NULL || kind < 0 || kind >= WF_K_COUNT) return wf_fail(WF_E_ARG, "bad argument");
     if (total_ms) *total_ms = s->k_ms[kind];
     if (launches) *launches = s->k_count[kind];
     return WF_OK;
 }
 
-extern "C" int wf_device_synchronize(void) {
-    WF_HIP(hipDeviceSynchronize());
-    return WF_OK;
-}
-
 /* last pressure-iteration count and error, for the step-loop driver */
-extern "C" int wf_solver_last_pressure(const wf_solver *s, double *error, int *iterations) {
+int last_pressure(const Solver *s, double *error, int *iterations) {
     if (s == NULL) return wf_fail(WF_E_ARG, "null solver");
     if (error) *error = s->last_err;
     if (iterations) *iterations = s->last_iters;
@@ -787,56 +825,7 @@ extern "C" int wf_solver_last_pressure(const wf_solver *s, double *error, int *i
 
 /* ------------------------------------------------------- learned closure ---- */
 
-static_assert(WF_CL_NPARAMS == WF_CLOSURE_NPARAMS, "closure parameter block differs from the public layout");
-
-/* float -> bf16 bits, round to nearest even (NaN kept quiet) */
-uint16_t wf_bf16_bits(float f) {
-    uint32_t u;
-    memcpy(&u, &f, 4);
-    if ((u & 0x7f800000u) == 0x7f800000u && (u & 0x007fffffu)) return (uint16_t)((u >> 16) | 0x40);
-    u += 0x7fffu + ((u >> 16) & 1u);
-    return (uint16_t)(u >> 16);
-}
-
-/* The MFMA B-operand fragments of the three layers (layout in wf_closure.h). */
-static void wf_closure_fragments(const float *prm, std::vector<uint16_t> &frag) {
-    frag.assign((size_t)WF_CL_NFRAG * 8, 0);
-    for (int l = 0; l < 64; l++) {
-        const int lr = l & 15, lg = l >> 4;
-        for (int j = 0; j < 8; j++) {
-            for (int nt = 0; nt < 4; nt++) {
-                const int n = nt * 16 + lr, k = 8 * lg + j;
-                const float w1 = (k < WF_CL_NIN) ? prm[WF_CL_W1 + n * WF_CL_NIN + k] : 0.0f;
-                frag[((size_t)(WF_CL_FRAG_W1 + nt * 64 + l)) * 8 + j] = wf_bf16_bits(w1);
-                for (int ks = 0; ks < 2; ks++) {
-                    const int k2 = ks * 32 + 8 * lg + j;
-                    frag[((size_t)(WF_CL_FRAG_W2 + (nt * 2 + ks) * 64 + l)) * 8 + j] =
-                        wf_bf16_bits(prm[WF_CL_W2 + n * WF_CL_NH + k2]);
-                }
-            }
-            for (int ks = 0; ks < 2; ks++) {
-                const int k3 = ks * 32 + 8 * lg + j;
-                frag[((size_t)(WF_CL_FRAG_W3 + ks * 64 + l)) * 8 + j] =
-                    wf_bf16_bits(lr == 0 ? prm[WF_CL_W3 + k3] : 0.0f);
-            }
-        }
-    }
-}
-
-static int wf_closure_upload(const float *params, long n, float **prm_dev, wf_bf16x8 **frag_dev) {
-    if (n != WF_CL_NPARAMS) return wf_fail(WF_E_ARG, "closure: %ld parameters given, %d expected", n, WF_CL_NPARAMS);
-    for (long q = 0; q < n; q++)
-        if (!std::isfinite(params[q])) return wf_fail(WF_E_ARG, "closure: parameter %ld is not finite", q);
-    std::vector<uint16_t> frag;
-    wf_closure_fragments(params, frag);
-    WF_HIP(hipMalloc((void **)prm_dev, sizeof(float) * n));
-    WF_HIP(hipMemcpy(*prm_dev, params, sizeof(float) * n, hipMemcpyHostToDevice));
-    WF_HIP(hipMalloc((void **)frag_dev, sizeof(wf_bf16x8) * WF_CL_NFRAG));
-    WF_HIP(hipMemcpy(*frag_dev, frag.data(), sizeof(wf_bf16x8) * WF_CL_NFRAG, hipMemcpyHostToDevice));
-    return WF_OK;
-}
-
-extern "C" int wf_solver_set_closure(wf_solver *s, const float *params, long n) {
+int set_closure(Solver *s, const float *params, long n) {
     if (s == NULL) return wf_fail(WF_E_ARG, "null solver");
     WF_HIP(hipStreamSynchronize(s->stream));
     if (s->cl_prm) WF_HIP(hipFree(s->cl_prm));
@@ -845,38 +834,10 @@ extern "C" int wf_solver_set_closure(wf_solver *s, const float *params, long n) 
     s->cl_frag = NULL;
     s->closure_on = 0;
     if (params == NULL) return WF_OK;
-    int rc = wf_closure_upload(params, n, &s->cl_prm, &s->cl_frag);
+    int rc = wf_closure_upload(params, n, &s->cl_prm, (void **)&s->cl_frag);
     if (rc) return rc;
     s->closure_on = 1;
     return WF_OK;
 }
 
-extern "C" int wf_closure_eval(const float *params, long nparams, const double *features, long n, double *nu_t,
-                               int *learned) {
-    if (params == NULL || features == NULL || nu_t == NULL || learned == NULL || n < 0)
-        return wf_fail(WF_E_ARG, "null or negative argument");
-    if (n == 0) return WF_OK;
-    int ndev = 0;
-    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return wf_fail(WF_E_DEVICE, "no HIP device");
-    WF_HIP(hipSetDevice(0));
-    float *prm = NULL;
-    wf_bf16x8 *frag = NULL;
-    double *f = NULL, *nu = NULL;
-    int *lf = NULL;
-    int rc = wf_closure_upload(params, nparams, &prm, &frag);
-    hipError_t e = hipSuccess;
-    if (rc == WF_OK) {
-        e = hipMalloc((void **)&f, sizeof(double) * WF_CL_NIN * n);
-        if (e == hipSuccess) e = hipMalloc((void **)&nu, sizeof(double) * n);
-        if (e == hipSuccess) e = hipMalloc((void **)&lf, sizeof(int) * n);
-        if (e == hipSuccess) e = hipMemcpy(f, features, sizeof(double) * WF_CL_NIN * n, hipMemcpyHostToDevice);
-        if (e == hipSuccess) e = wf_launch_closure_eval(WfClosure{prm, frag}, f, n, nu, lf, 0);
-        if (e == hipSuccess) e = hipMemcpy(nu_t, nu, sizeof(double) * n, hipMemcpyDeviceToHost);
-        if (e == hipSuccess) e = hipMemcpy(learned, lf, sizeof(int) * n, hipMemcpyDeviceToHost);
-        if (e != hipSuccess) rc = wf_fail(WF_E_DEVICE, "closure evaluation: %s", hipGetErrorString(e));
-    }
-    void *bufs[] = {prm, frag, f, nu, lf};
-    for (void *b : bufs)
-        if (b) (void)hipFree(b);
-    return rc;
-}
+}  // namespace WF_NS

@@ -22,33 +22,36 @@
 #include "wf_kernels.h"
 #include "wf_closure.h"
 
+namespace WF_NS {
+
 /* 32-bit element offsets from a uniform base (global loads with a scalar base
  * and a 32-bit lane offset; the solver limits Nx*Ny*Nz to 2^28). */
-__device__ __forceinline__ double wf_ld(const double *base, unsigned q) {
-    return *(const double *)((const char *)base + (q << 3));
+template <typename T>
+__device__ __forceinline__ T wf_ld(const T *base, unsigned q) {
+    return *(const T *)((const char *)base + q * (unsigned)sizeof(T));
 }
 
 /* Heat source q of the temperature RHS (src/cu/pde.cu:663-699): combustion
  * H_C Y K(T)/c_p minus the gas-solid exchange, with the gas/solid temperatures
  * of the fuel zone k <= Nz_Y (the exchange term is formed only when
  * alpha_s sigma_s != 0, see wf_heat_source_y). */
-__device__ __forceinline__ double wf_heat_q(const WfConst &c, int k, int nzy, double T, double T_kp1, double Ynode,
-                                            double K_T, double dz_k, double *Yout) {
+__device__ __forceinline__ wf_real wf_heat_q(const WfConst &c, int k, int nzy, wf_real T, wf_real T_kp1, wf_real Ynode,
+                                            wf_real K_T, wf_real dz_k, wf_real *Yout) {
 #pragma clang fp contract(fast)
-    double T_gas = c.T_inf, T_solid = c.T_inf, Y = 0.0;
+    wf_real T_gas = c.T_inf, T_solid = c.T_inf, Y = WR(0.0);
     if (k <= nzy) {
         if (k < nzy)
             Y = Ynode;
         else
             T_gas = T_kp1;
-        if (Y > 0.0)
+        if (Y > WR(0.0))
             T_solid = T;
         else
             T_gas = T;
     }
-    double q = c.H_C * Y * K_T * c.inv_cp;
+    wf_real q = c.H_C * Y * K_T * c.inv_cp;
     if (c.exchange_on) {
-        double h_c = c.h_c * dz_k * pow(fabs(T_gas - T_solid) / dz_k, 1.0 / 4.0);
+        wf_real h_c = c.h_c * dz_k * pow(fabs(T_gas - T_solid) / dz_k, WR(1.0) / WR(4.0));
         q = q - h_c * c.alpha_s * c.sigma_s * (T_gas - T_solid) / (c.c_p * c.T_inf * c.rho_inf / T_gas);
     }
     *Yout = Y;
@@ -72,7 +75,7 @@ __global__ __launch_bounds__(256) void k_derived(WfConst c, WfState X, WfDerived
         if (!inside) return;
     }
     const int Nx = c.Nx, Ny = c.Ny, Nz = c.Nz;
-    const double *zk = c.zc + WF_ZC * k;
+    const wf_real *zk = c.zc + WF_ZC * k;
     const int im1 = wf_wrap_m(i, 1, Nx), ip1 = wf_wrap_p(i, 1, Nx);
     const int jm1 = wf_wrap_m(j, 1, Ny), jp1 = wf_wrap_p(j, 1, Ny);
     const unsigned P = (unsigned)c.plane;
@@ -83,28 +86,28 @@ __global__ __launch_bounds__(256) void k_derived(WfConst c, WfState X, WfDerived
     const unsigned km1 = (k >= 1) ? q - P : q, km2 = (k >= 2) ? q - 2 * P : q, km3 = (k >= 3) ? q - 3 * P : q;
     const unsigned kp1 = (k <= Nz - 2) ? q + P : q, kp2 = (k <= Nz - 3) ? q + 2 * P : q,
                    kp3 = (k <= Nz - 4) ? q + 3 * P : q;
-    const double u = wf_ld(X.u, q), v = wf_ld(X.v, q), w = wf_ld(X.w, q), T = wf_ld(X.T, q);
-    const double ux = (wf_ld(X.u, qxp) - wf_ld(X.u, qxm)) * c.inv2dx, uy = (wf_ld(X.u, qyp) - wf_ld(X.u, qym)) * c.inv2dy;
-    const double vx = (wf_ld(X.v, qxp) - wf_ld(X.v, qxm)) * c.inv2dx, vy = (wf_ld(X.v, qyp) - wf_ld(X.v, qym)) * c.inv2dy;
-    const double wx = (wf_ld(X.w, qxp) - wf_ld(X.w, qxm)) * c.inv2dx, wy = (wf_ld(X.w, qyp) - wf_ld(X.w, qym)) * c.inv2dy;
-    const double Txm = wf_ld(X.T, qxm), Txp = wf_ld(X.T, qxp), Tym = wf_ld(X.T, qym), Typ = wf_ld(X.T, qyp);
-    const double Tx = (Txp - Txm) * c.inv2dx, Ty = (Typ - Tym) * c.inv2dy;
-    const double Txx = (Txp - 2.0 * T + Txm) * c.invdx2;
-    const double Tyy = (Typ - 2.0 * T + Tym) * c.invdy2;
-    const double u1 = wf_ld(X.u, kp1), u2 = wf_ld(X.u, kp2), um1 = wf_ld(X.u, km1), um2 = wf_ld(X.u, km2);
-    const double v1 = wf_ld(X.v, kp1), v2 = wf_ld(X.v, kp2), vm1 = wf_ld(X.v, km1), vm2 = wf_ld(X.v, km2);
-    const double w1 = wf_ld(X.w, kp1), w2 = wf_ld(X.w, kp2), wm1 = wf_ld(X.w, km1), wm2 = wf_ld(X.w, km2);
-    const double T1 = wf_ld(X.T, kp1), T2z = wf_ld(X.T, kp2), T3z = wf_ld(X.T, kp3);
-    const double Tm1 = wf_ld(X.T, km1), Tm2 = wf_ld(X.T, km2), Tm3 = wf_ld(X.T, km3);
+    const wf_real u = wf_ld(X.u, q), v = wf_ld(X.v, q), w = wf_ld(X.w, q), T = wf_ld(X.T, q);
+    const wf_real ux = (wf_ld(X.u, qxp) - wf_ld(X.u, qxm)) * c.inv2dx, uy = (wf_ld(X.u, qyp) - wf_ld(X.u, qym)) * c.inv2dy;
+    const wf_real vx = (wf_ld(X.v, qxp) - wf_ld(X.v, qxm)) * c.inv2dx, vy = (wf_ld(X.v, qyp) - wf_ld(X.v, qym)) * c.inv2dy;
+    const wf_real wx = (wf_ld(X.w, qxp) - wf_ld(X.w, qxm)) * c.inv2dx, wy = (wf_ld(X.w, qyp) - wf_ld(X.w, qym)) * c.inv2dy;
+    const wf_real Txm = wf_ld(X.T, qxm), Txp = wf_ld(X.T, qxp), Tym = wf_ld(X.T, qym), Typ = wf_ld(X.T, qyp);
+    const wf_real Tx = (Txp - Txm) * c.inv2dx, Ty = (Typ - Tym) * c.inv2dy;
+    const wf_real Txx = (Txp - WR(2.0) * T + Txm) * c.invdx2;
+    const wf_real Tyy = (Typ - WR(2.0) * T + Tym) * c.invdy2;
+    const wf_real u1 = wf_ld(X.u, kp1), u2 = wf_ld(X.u, kp2), um1 = wf_ld(X.u, km1), um2 = wf_ld(X.u, km2);
+    const wf_real v1 = wf_ld(X.v, kp1), v2 = wf_ld(X.v, kp2), vm1 = wf_ld(X.v, km1), vm2 = wf_ld(X.v, km2);
+    const wf_real w1 = wf_ld(X.w, kp1), w2 = wf_ld(X.w, kp2), wm1 = wf_ld(X.w, km1), wm2 = wf_ld(X.w, km2);
+    const wf_real T1 = wf_ld(X.T, kp1), T2z = wf_ld(X.T, kp2), T3z = wf_ld(X.T, kp3);
+    const wf_real Tm1 = wf_ld(X.T, km1), Tm2 = wf_ld(X.T, km2), Tm3 = wf_ld(X.T, km3);
     /* the fuel column and Y with the other loads: Y keeps Nz_Y_max >= nzy
      * planes, so the load is unconditional (the k = 0 entry above them, T
      * when there is no Y) and selected afterwards */
     const int col = i + Nx * j;
     const int nzy = c.nzy[col];
-    const double Yld = wf_ld((c.Nzm > 0) ? X.Y : X.T, (k < c.Nzm) ? q : (unsigned)col);
-    double uz, vz, wz, Tz, Tzz, T_kp1 = 0.0;
+    const double Yld = wf_ld(X.Y, (k < c.Nzm) ? q : (unsigned)col);
+    wf_real uz, vz, wz, Tz, Tzz, T_kp1 = WR(0.0);
     if (k == 0) { /* one-sided (src/cu/pde.cu:363-365, 497-500) */
-        const double *f = zk + WF_Z_D1F, *g = zk + WF_Z_D2F;
+        const wf_real *f = zk + WF_Z_D1F, *g = zk + WF_Z_D2F;
         uz = f[0] * u + f[1] * u1 + f[2] * u2;
         vz = f[0] * v + f[1] * v1 + f[2] * v2;
         wz = f[0] * w + f[1] * w1 + f[2] * w2;
@@ -112,14 +115,14 @@ __global__ __launch_bounds__(256) void k_derived(WfConst c, WfState X, WfDerived
         Tz = f[0] * T + f[1] * T1 + f[2] * T2z;
         Tzz = g[0] * T + g[1] * T1 + g[2] * T2z + g[3] * T3z;
     } else if (k == Nz - 1) { /* src/cu/pde.cu:383-385, 539-542 */
-        const double *b = zk + WF_Z_D1B, *g = zk + WF_Z_D2B;
+        const wf_real *b = zk + WF_Z_D1B, *g = zk + WF_Z_D2B;
         uz = b[0] * um2 + b[1] * um1 + b[2] * u;
         vz = b[0] * vm2 + b[1] * vm1 + b[2] * v;
         wz = b[0] * wm2 + b[1] * wm1 + b[2] * w;
         Tz = b[0] * Tm2 + b[1] * Tm1 + b[2] * T;
         Tzz = g[0] * Tm3 + g[1] * Tm2 + g[2] * Tm1 + g[3] * T;
     } else {
-        const double dzc = zk[WF_Z_DZC], *m = zk + WF_Z_D2M;
+        const wf_real dzc = zk[WF_Z_DZC], *m = zk + WF_Z_D2M;
         uz = (u1 - um1) * dzc;
         vz = (v1 - vm1) * dzc;
         wz = (w1 - wm1) * dzc;
@@ -127,36 +130,36 @@ __global__ __launch_bounds__(256) void k_derived(WfConst c, WfState X, WfDerived
         Tz = (T1 - Tm1) * dzc;
         Tzz = m[0] * Tm1 + m[1] * T + m[2] * T1;
     }
-    const double lap_T = Txx + Tyy + Tzz;
-    const double invT = 1.0 / T;
-    const double rho = c.T_inf * c.rho_inf * invT;
-    const double inv_rho = T * c.inv_Trho;
-    const double third = 1.0 / 3.0;
-    const double s11 = (2 * ux) * third - (vy + wz) * third;
-    const double s22 = (2 * vy) * third - (ux + wz) * third;
-    const double s33 = (2 * wz) * third - (ux + vy) * third;
-    const double s12 = 0.5 * (uy + vx);
-    const double s13 = 0.5 * (uz + wx);
-    const double s23 = 0.5 * (wy + vz);
-    const double mod_S = sqrt(2.0 * (ux * ux + vy * vy + wz * wz) + (uz + wx) * (uz + wx) + (vx + uy) * (vx + uy) +
+    const wf_real lap_T = Txx + Tyy + Tzz;
+    const wf_real invT = WR(1.0) / T;
+    const wf_real rho = c.T_inf * c.rho_inf * invT;
+    const wf_real inv_rho = T * c.inv_Trho;
+    const wf_real third = WR(1.0) / WR(3.0);
+    const wf_real s11 = (2 * ux) * third - (vy + wz) * third;
+    const wf_real s22 = (2 * vy) * third - (ux + wz) * third;
+    const wf_real s33 = (2 * wz) * third - (ux + vy) * third;
+    const wf_real s12 = WR(0.5) * (uy + vx);
+    const wf_real s13 = WR(0.5) * (uz + wx);
+    const wf_real s23 = WR(0.5) * (wy + vz);
+    const wf_real mod_S = sqrt(WR(2.0) * (ux * ux + vy * vy + wz * wz) + (uz + wx) * (uz + wx) + (vx + uy) * (vx + uy) +
                               (wy + vz) * (wy + vz));
-    double mu_sgs = rho * c.cd2[k] * mod_S;
+    wf_real mu_sgs = rho * c.cd2[k] * mod_S;
     /* T_RHS_tmp of src/cu/pde.cu:706 (needed for div_U) */
-    double mu = c.mu, kT = c.kT, dk = 0.0;
-    const double K_T = (T > c.T_pc) ? c.A * exp(-c.T_act * invT) : 0.0;
-    const double Ynode = (k < nzy) ? Yld : 0.0;
-    const double dz_k = (k < Nz - 1) ? c.z[k + 1] - c.z[k] : c.z[k] - c.z[k - 1];
-    double Y;
-    const double q_src = wf_heat_q(c, k, nzy, T, T_kp1, Ynode, K_T, dz_k, &Y);
-    if (c.suth_mu == 1) mu = 1.716e-5 * pow(T / 273.15, 1.5) * (273.15 + 110.4) / (T + 110.4);
+    wf_real mu = c.mu, kT = c.kT, dk = WR(0.0);
+    const wf_real K_T = (T > c.T_pc) ? c.A * exp(-c.T_act * invT) : WR(0.0);
+    const wf_real Ynode = (k < nzy) ? (wf_real)Yld : WR(0.0);
+    const wf_real dz_k = (k < Nz - 1) ? c.z[k + 1] - c.z[k] : c.z[k] - c.z[k - 1];
+    wf_real Y;
+    const wf_real q_src = wf_heat_q(c, k, nzy, T, T_kp1, Ynode, K_T, dz_k, &Y);
+    if (c.suth_mu == 1) mu = WR(1.716e-5) * pow(T / WR(273.15), WR(1.5)) * (WR(273.15) + WR(110.4)) / (T + WR(110.4));
     if (c.suth_k == 1) wf_sutherland_k(T, &kT, &dk);
-    const double nu = mu * inv_rho;
-    const double T2 = T * T, T3 = T * T * T;
-    const double T_RHS_tmp = (dk + 12 * WF_SIGMA_D * c.delta * T2 * (Tx * Tx + Ty * Ty + Tz * Tz) +
-                              (kT + 4 * WF_SIGMA_D * c.delta * T3) * lap_T) * inv_rho * c.inv_cp + q_src;
+    const wf_real nu = mu * inv_rho;
+    const wf_real T2 = T * T, T3 = T * T * T;
+    const wf_real T_RHS_tmp = (dk + 12 * WR(WF_SIGMA_D) * c.delta * T2 * (Tx * Tx + Ty * Ty + Tz * Tz) +
+                               (kT + 4 * WR(WF_SIGMA_D) * c.delta * T3) * lap_T) * inv_rho * c.inv_cp + q_src;
     if constexpr (CL) {
         __shared__ WfClosureLds cl_lds[WF_TILE_Y];
-        const double f[WF_CL_NIN] = {ux, uy, uz, vx, vy, vz, wx, wy, wz,
+        const wf_real f[WF_CL_NIN] = {ux, uy, uz, vx, vy, vz, wx, wy, wz,
                                      sqrt(Tx * Tx + Ty * Ty + Tz * Tz), c.delta_k[k], Ynode};
         __bf16 x[WF_CL_NIN];
         const bool in_range = wf_cl_inputs(cl.prm, f, x);
@@ -173,8 +176,8 @@ __global__ __launch_bounds__(256) void k_derived(WfConst c, WfState X, WfDerived
     D.s33[q] = s33;
     D.divU[q] = nu * T_RHS_tmp * invT;
     if (k == 0) {
-        const double a1 = mu * (uz + wx * 0.0), a2 = mu * (vz + wy * 0.0);
-        const double tau_w = sqrt(a1 * a1 + a2 * a2);
+        const wf_real a1 = mu * (uz + wx * WR(0.0)), a2 = mu * (vz + wy * WR(0.0));
+        const wf_real tau_w = sqrt(a1 * a1 + a2 * a2);
         D.utau[col] = sqrt(tau_w * inv_rho);
     }
 }
@@ -204,10 +207,10 @@ __global__ __launch_bounds__(256) void k_derived(WfConst c, WfState X, WfDerived
 #endif
 
 struct WfStageTiles {
-    double st[4][WF_SHH * WF_SW]; /* u, v, w, T */
-    double dv[7][WF_DHH * WF_DW]; /* mu, s11, s12, s13, s22, s23, divU */
-    double ut[WF_DHH * WF_DW];    /* u_tau (k = 0 plane) */
-    double tp[WF_DHH * WF_DW];    /* topography */
+    wf_real st[4][WF_SHH * WF_SW]; /* u, v, w, T */
+    wf_real dv[7][WF_DHH * WF_DW]; /* mu, s11, s12, s13, s22, s23, divU */
+    wf_real ut[WF_DHH * WF_DW];    /* u_tau (k = 0 plane) */
+    wf_real tp[WF_DHH * WF_DW];    /* topography */
 };
 
 /* Periodic neighbour of tile position g (= i0 - halo + offset) with the
@@ -224,7 +227,7 @@ __device__ __forceinline__ int wf_tile_wrap(int g, int m) {
  * into registers; all lanes load (out-of-tile entries read node 0) so the
  * loads issue without branches. */
 template <int W, int HH, int H>
-__device__ __forceinline__ void wf_tile_load(double *v, const double *src, const WfConst &c, int i0, int j0, int k,
+__device__ __forceinline__ void wf_tile_load(wf_real *v, const wf_real *src, const WfConst &c, int i0, int j0, int k,
                                              int tid) {
     constexpr int NE = W * HH;
     constexpr int PER = (NE + WF_NT_STAGE - 1) / WF_NT_STAGE;
@@ -240,7 +243,7 @@ __device__ __forceinline__ void wf_tile_load(double *v, const double *src, const
 }
 
 template <int W, int HH>
-__device__ __forceinline__ void wf_tile_store(double *dst, const double *v, int tid) {
+__device__ __forceinline__ void wf_tile_store(wf_real *dst, const wf_real *v, int tid) {
     constexpr int NE = W * HH;
     constexpr int PER = (NE + WF_NT_STAGE - 1) / WF_NT_STAGE;
 #pragma unroll
@@ -253,10 +256,10 @@ __device__ __forceinline__ void wf_tile_store(double *dst, const double *v, int 
 /* The node's own column: u, v, w, T at k-2..k+2 (index 2 = k) and the
  * derived fields the SGS divergence reads above and below. */
 struct WfColumnVals {
-    double s[4][5];   /* u, v, w, T at k-2, k-1, k, k+1, k+2 (index 2 = k) */
-    double mu[3], s13[3], s23[3], s33[3], dU[3]; /* k-1, k, k+1 */
-    double s11, s12, s22;
-    double Y, utau, topo;
+    wf_real s[4][5];   /* u, v, w, T at k-2, k-1, k, k+1, k+2 (index 2 = k) */
+    wf_real mu[3], s13[3], s23[3], s33[3], dU[3]; /* k-1, k, k+1 */
+    wf_real s11, s12, s22;
+    wf_real Y, utau, topo;
 };
 
 /* Derived-field indices of the accessors (the first seven are the LDS tiles). */
@@ -273,10 +276,10 @@ struct WfTileAcc {
     const WfStageTiles &L;
     const WfColumnVals &V;
     int sc, dc;
-    __device__ __forceinline__ double st(int f, int di, int dj) const { return L.st[f][sc + di + dj * WF_SW]; }
-    __device__ __forceinline__ double sz(int f, int dk) const { return V.s[f][2 + dk]; }
-    __device__ __forceinline__ double dv(int f, int di, int dj) const { return L.dv[f][dc + di + dj * WF_DW]; }
-    __device__ __forceinline__ double dz(int f, int dk) const {
+    __device__ __forceinline__ wf_real st(int f, int di, int dj) const { return L.st[f][sc + di + dj * WF_SW]; }
+    __device__ __forceinline__ wf_real sz(int f, int dk) const { return V.s[f][2 + dk]; }
+    __device__ __forceinline__ wf_real dv(int f, int di, int dj) const { return L.dv[f][dc + di + dj * WF_DW]; }
+    __device__ __forceinline__ wf_real dz(int f, int dk) const {
         switch (f) {
             case WF_D_MU: return V.mu[1 + dk];
             case WF_D_S13: return V.s13[1 + dk];
@@ -288,13 +291,13 @@ struct WfTileAcc {
             default: return V.s22;
         }
     }
-    __device__ __forceinline__ double ut(int di, int dj) const {
+    __device__ __forceinline__ wf_real ut(int di, int dj) const {
         return (di == 0 && dj == 0) ? V.utau : L.ut[dc + di + dj * WF_DW];
     }
-    __device__ __forceinline__ double tp(int di, int dj) const {
+    __device__ __forceinline__ wf_real tp(int di, int dj) const {
         return (di == 0 && dj == 0) ? V.topo : L.tp[dc + di + dj * WF_DW];
     }
-    __device__ __forceinline__ double Y() const { return V.Y; }
+    __device__ __forceinline__ wf_real Y() const { return V.Y; }
 };
 
 struct WfGlobalAcc {
@@ -308,10 +311,10 @@ struct WfGlobalAcc {
     __device__ __forceinline__ int nj(int dj) const {
         return (dj > 0) ? wf_wrap_p(j, dj, c.Ny) : (dj < 0) ? wf_wrap_m(j, -dj, c.Ny) : j;
     }
-    __device__ __forceinline__ const double *sfield(int f) const {
+    __device__ __forceinline__ const wf_real *sfield(int f) const {
         return (f == 0) ? X.u : (f == 1) ? X.v : (f == 2) ? X.w : X.T;
     }
-    __device__ __forceinline__ const double *dfield(int f) const {
+    __device__ __forceinline__ const wf_real *dfield(int f) const {
         switch (f) {
             case WF_D_MU: return D.mu;
             case WF_D_S11: return D.s11;
@@ -323,22 +326,22 @@ struct WfGlobalAcc {
             default: return D.s33;
         }
     }
-    __device__ __forceinline__ double st(int f, int di, int dj) const {
+    __device__ __forceinline__ wf_real st(int f, int di, int dj) const {
         return wf_ld(sfield(f), (unsigned)wf_q(c, ni(di), nj(dj), k));
     }
-    __device__ __forceinline__ double sz(int f, int dk) const {
+    __device__ __forceinline__ wf_real sz(int f, int dk) const {
         int kk = k + dk;
         kk = (kk < 0) ? 0 : (kk > c.Nz - 1) ? c.Nz - 1 : kk;
         return wf_ld(sfield(f), (unsigned)wf_q(c, i, j, kk));
     }
-    __device__ __forceinline__ double dv(int f, int di, int dj) const {
+    __device__ __forceinline__ wf_real dv(int f, int di, int dj) const {
         return wf_ld(dfield(f), (unsigned)wf_q(c, ni(di), nj(dj), k));
     }
-    __device__ __forceinline__ double dz(int f, int dk) const { return wf_ld(dfield(f), (unsigned)wf_q(c, i, j, k + dk)); }
-    __device__ __forceinline__ double ut(int di, int dj) const { return D.utau[ni(di) + c.Nx * nj(dj)]; }
-    __device__ __forceinline__ double tp(int di, int dj) const { return c.topo[ni(di) + c.Nx * nj(dj)]; }
-    __device__ __forceinline__ double Y() const {
-        return (k < c.Nzm) ? wf_ld(X.Y, (unsigned)wf_q(c, i, j, k)) : 0.0;
+    __device__ __forceinline__ wf_real dz(int f, int dk) const { return wf_ld(dfield(f), (unsigned)wf_q(c, i, j, k + dk)); }
+    __device__ __forceinline__ wf_real ut(int di, int dj) const { return D.utau[ni(di) + c.Nx * nj(dj)]; }
+    __device__ __forceinline__ wf_real tp(int di, int dj) const { return c.topo[ni(di) + c.Nx * nj(dj)]; }
+    __device__ __forceinline__ wf_real Y() const {
+        return (k < c.Nzm) ? (wf_real)wf_ld(X.Y, (unsigned)wf_q(c, i, j, k)) : WR(0.0);
     }
 };
 
@@ -346,39 +349,39 @@ struct WfGlobalAcc {
  * interior node 1 <= k <= Nz-2 (src/cu/pde.cu:139-787,
  * src/cu/turbulence.cu:255-454). */
 template <bool CL, class Acc>
-__device__ __forceinline__ void wf_rhs_node(const WfConst &c, const Acc &a, int k, int nzy, double r[5]) {
+__device__ __forceinline__ void wf_rhs_node(const WfConst &c, const Acc &a, int k, int nzy, wf_real r[5]) {
 #pragma clang fp contract(fast)
     const int Nz = c.Nz;
-    const double *zk = c.zc + WF_ZC * k;
-    const double *d1f = zk + WF_Z_D1F, *d1b = zk + WF_Z_D1B, *d2m = zk + WF_Z_D2M;
-    const double dzc = zk[WF_Z_DZC];
-    const double u = a.sz(0, 0), v = a.sz(1, 0), w = a.sz(2, 0), T = a.sz(3, 0);
-    const double u_plu = (u > 0.0) ? u : 0.0, u_min = (u < 0.0) ? u : 0.0;
-    const double v_plu = (v > 0.0) ? v : 0.0, v_min = (v < 0.0) ? v : 0.0;
-    const double w_plu = (w > 0.0) ? w : 0.0, w_min = (w < 0.0) ? w : 0.0;
-    double adv[4], lap[4];
-    double Tx = 0.0, Ty = 0.0, Tz = 0.0, Txx = 0.0, Tyy = 0.0, Tzz = 0.0, T_kp1 = 0.0;
+    const wf_real *zk = c.zc + WF_ZC * k;
+    const wf_real *d1f = zk + WF_Z_D1F, *d1b = zk + WF_Z_D1B, *d2m = zk + WF_Z_D2M;
+    const wf_real dzc = zk[WF_Z_DZC];
+    const wf_real u = a.sz(0, 0), v = a.sz(1, 0), w = a.sz(2, 0), T = a.sz(3, 0);
+    const wf_real u_plu = (u > WR(0.0)) ? u : WR(0.0), u_min = (u < WR(0.0)) ? u : WR(0.0);
+    const wf_real v_plu = (v > WR(0.0)) ? v : WR(0.0), v_min = (v < WR(0.0)) ? v : WR(0.0);
+    const wf_real w_plu = (w > WR(0.0)) ? w : WR(0.0), w_min = (w < WR(0.0)) ? w : WR(0.0);
+    wf_real adv[4], lap[4];
+    wf_real Tx = WR(0.0), Ty = WR(0.0), Tz = WR(0.0), Txx = WR(0.0), Tyy = WR(0.0), Tzz = WR(0.0), T_kp1 = WR(0.0);
 #pragma unroll
     for (int f = 0; f < 4; f++) {
-        const double a0 = a.sz(f, 0);
-        const double am1 = a.st(f, -1, 0), am2 = a.st(f, -2, 0), ap1 = a.st(f, 1, 0), ap2 = a.st(f, 2, 0);
-        const double bm1 = a.st(f, 0, -1), bm2 = a.st(f, 0, -2), bp1 = a.st(f, 0, 1), bp2 = a.st(f, 0, 2);
-        const double cm2 = a.sz(f, -2), cm1 = a.sz(f, -1), cp1 = a.sz(f, 1), cp2 = a.sz(f, 2);
+        const wf_real a0 = a.sz(f, 0);
+        const wf_real am1 = a.st(f, -1, 0), am2 = a.st(f, -2, 0), ap1 = a.st(f, 1, 0), ap2 = a.st(f, 2, 0);
+        const wf_real bm1 = a.st(f, 0, -1), bm2 = a.st(f, 0, -2), bp1 = a.st(f, 0, 1), bp2 = a.st(f, 0, 2);
+        const wf_real cm2 = a.sz(f, -2), cm1 = a.sz(f, -1), cp1 = a.sz(f, 1), cp2 = a.sz(f, 2);
         /* second-order upwind differences (src/cu/pde.cu:355-451) */
-        const double d_im = (3 * a0 - 4 * am1 + am2) * c.inv2dx;
-        const double d_ip = (-3 * a0 + 4 * ap1 - ap2) * c.inv2dx;
-        const double d_jm = (3 * a0 - 4 * bm1 + bm2) * c.inv2dy;
-        const double d_jp = (-3 * a0 + 4 * bp1 - bp2) * c.inv2dy;
-        const double fwd = d1f[0] * a0 + d1f[1] * cp1 + d1f[2] * cp2;
-        const double bwd = d1b[0] * cm2 + d1b[1] * cm1 + d1b[2] * a0;
-        const double d_km = (k <= 1) ? fwd : bwd;
-        const double d_kp = (k >= Nz - 2) ? bwd : fwd;
-        const double dxx = (ap1 - 2.0 * a0 + am1) * c.invdx2;
-        const double dyy = (bp1 - 2.0 * a0 + bm1) * c.invdy2;
-        const double dzz = d2m[0] * cm1 + d2m[1] * a0 + d2m[2] * cp1;
-        const double ax = u_plu * d_im + u_min * d_ip;
-        const double ay = v_plu * d_jm + v_min * d_jp;
-        const double az = w_plu * d_km + w_min * d_kp;
+        const wf_real d_im = (3 * a0 - 4 * am1 + am2) * c.inv2dx;
+        const wf_real d_ip = (-3 * a0 + 4 * ap1 - ap2) * c.inv2dx;
+        const wf_real d_jm = (3 * a0 - 4 * bm1 + bm2) * c.inv2dy;
+        const wf_real d_jp = (-3 * a0 + 4 * bp1 - bp2) * c.inv2dy;
+        const wf_real fwd = d1f[0] * a0 + d1f[1] * cp1 + d1f[2] * cp2;
+        const wf_real bwd = d1b[0] * cm2 + d1b[1] * cm1 + d1b[2] * a0;
+        const wf_real d_km = (k <= 1) ? fwd : bwd;
+        const wf_real d_kp = (k >= Nz - 2) ? bwd : fwd;
+        const wf_real dxx = (ap1 - WR(2.0) * a0 + am1) * c.invdx2;
+        const wf_real dyy = (bp1 - WR(2.0) * a0 + bm1) * c.invdy2;
+        const wf_real dzz = d2m[0] * cm1 + d2m[1] * a0 + d2m[2] * cp1;
+        const wf_real ax = u_plu * d_im + u_min * d_ip;
+        const wf_real ay = v_plu * d_jm + v_min * d_jp;
+        const wf_real az = w_plu * d_km + w_min * d_kp;
         adv[f] = ax + ay + az;
         lap[f] = dxx + dyy + dzz;
         if (f == 3) {
@@ -391,127 +394,119 @@ __device__ __forceinline__ void wf_rhs_node(const WfConst &c, const Acc &a, int 
             T_kp1 = cp1;
         }
     }
-    const double invT = 1.0 / T;
-    const double rho = c.T_inf * c.rho_inf * invT;
-    const double inv_rho = T * c.inv_Trho;
-    const double K_T = (T > c.T_pc) ? c.A * exp(-c.T_act * invT) : 0.0;
-    double Y;
-    const double q_src = wf_heat_q(c, k, nzy, T, T_kp1, a.Y(), K_T, c.z[k + 1] - c.z[k], &Y);
-    double mu = c.mu, kT = c.kT, dk = 0.0;
-    if (c.suth_mu == 1) mu = 1.716e-5 * pow(T / 273.15, 1.5) * (273.15 + 110.4) / (T + 110.4);
+    const wf_real invT = WR(1.0) / T;
+    const wf_real rho = c.T_inf * c.rho_inf * invT;
+    const wf_real inv_rho = T * c.inv_Trho;
+    const wf_real K_T = (T > c.T_pc) ? c.A * exp(-c.T_act * invT) : WR(0.0);
+    wf_real Y;
+    const wf_real q_src = wf_heat_q(c, k, nzy, T, T_kp1, a.Y(), K_T, c.z[k + 1] - c.z[k], &Y);
+    wf_real mu = c.mu, kT = c.kT, dk = WR(0.0);
+    if (c.suth_mu == 1) mu = WR(1.716e-5) * pow(T / WR(273.15), WR(1.5)) * (WR(273.15) + WR(110.4)) / (T + WR(110.4));
     if (c.suth_k == 1) wf_sutherland_k(T, &kT, &dk);
-    const double nu = mu * inv_rho;
-    const double T2 = T * T, T3 = T * T * T;
-    const double T_RHS_tmp = (dk + 12 * WF_SIGMA_D * c.delta * T2 * (Tx * Tx + Ty * Ty + Tz * Tz) +
-                              (kT + 4 * WF_SIGMA_D * c.delta * T3) * lap[3]) * inv_rho * c.inv_cp + q_src;
-    const double T_RHS = (c.temperature_convection == 0) ? T_RHS_tmp - (u * Tx + v * Ty + w * Tz) : T_RHS_tmp - adv[3];
-    const double mod_U = sqrt(u * u + v * v + w * w);
-    const double drag = -0.5 * c.C_d * c.alpha_s * c.sigma_s * Y * mod_U;
-    const double F_x = drag * u;
-    const double F_y = drag * v;
-    const double F_z = c.g * (rho - c.rho_inf) * inv_rho + drag * w;
-    const double u_RHS = nu * lap[0] - adv[0] + F_x;
-    const double v_RHS = nu * lap[1] - adv[1] + F_y;
-    const double w_RHS = nu * lap[2] - adv[2] + F_z;
+    const wf_real nu = mu * inv_rho;
+    const wf_real T2 = T * T, T3 = T * T * T;
+    const wf_real T_RHS_tmp = (dk + 12 * WR(WF_SIGMA_D) * c.delta * T2 * (Tx * Tx + Ty * Ty + Tz * Tz) +
+                               (kT + 4 * WR(WF_SIGMA_D) * c.delta * T3) * lap[3]) * inv_rho * c.inv_cp + q_src;
+    const wf_real T_RHS = (c.temperature_convection == 0) ? T_RHS_tmp - (u * Tx + v * Ty + w * Tz) : T_RHS_tmp - adv[3];
+    const wf_real mod_U = sqrt(u * u + v * v + w * w);
+    const wf_real drag = -WR(0.5) * c.C_d * c.alpha_s * c.sigma_s * Y * mod_U;
+    const wf_real F_x = drag * u;
+    const wf_real F_y = drag * v;
+    const wf_real F_z = c.g * (rho - c.rho_inf) * inv_rho + drag * w;
+    const wf_real u_RHS = nu * lap[0] - adv[0] + F_x;
+    const wf_real v_RHS = nu * lap[1] - adv[1] + F_y;
+    const wf_real w_RHS = nu * lap[2] - adv[2] + F_z;
 
     /* SGS divergence (src/cu/turbulence.cu:291-452): mu_sgs damped by
      * fw^2 = (1 - exp(-z_ibm u_tau / (A_VD nu)))^2 at the node and its six
      * neighbours, nu the node's own c.mu / rho */
-    const double inv_nu_t = rho * c.inv_mu;
-    const double ut0 = a.ut(0, 0), tp0 = a.tp(0, 0);
-    const double fw = wf_fw_inv(wf_zibm_t(c, tp0, k), ut0, inv_nu_t);
-    const double fw_ip = wf_fw_inv(wf_zibm_t(c, a.tp(1, 0), k), a.ut(1, 0), inv_nu_t);
-    const double fw_im = wf_fw_inv(wf_zibm_t(c, a.tp(-1, 0), k), a.ut(-1, 0), inv_nu_t);
-    const double fw_jp = wf_fw_inv(wf_zibm_t(c, a.tp(0, 1), k), a.ut(0, 1), inv_nu_t);
-    const double fw_jm = wf_fw_inv(wf_zibm_t(c, a.tp(0, -1), k), a.ut(0, -1), inv_nu_t);
-    const double fw_km = wf_fw_inv(wf_zibm_t(c, tp0, k - 1), ut0, inv_nu_t);
-    const double fw_kp = wf_fw_inv(wf_zibm_t(c, tp0, k + 1), ut0, inv_nu_t);
+    const wf_real inv_nu_t = rho * c.inv_mu;
+    const wf_real ut0 = a.ut(0, 0), tp0 = a.tp(0, 0);
+    const wf_real fw = wf_fw_inv(wf_zibm_t(c, tp0, k), ut0, inv_nu_t);
+    const wf_real fw_ip = wf_fw_inv(wf_zibm_t(c, a.tp(1, 0), k), a.ut(1, 0), inv_nu_t);
+    const wf_real fw_im = wf_fw_inv(wf_zibm_t(c, a.tp(-1, 0), k), a.ut(-1, 0), inv_nu_t);
+    const wf_real fw_jp = wf_fw_inv(wf_zibm_t(c, a.tp(0, 1), k), a.ut(0, 1), inv_nu_t);
+    const wf_real fw_jm = wf_fw_inv(wf_zibm_t(c, a.tp(0, -1), k), a.ut(0, -1), inv_nu_t);
+    const wf_real fw_km = wf_fw_inv(wf_zibm_t(c, tp0, k - 1), ut0, inv_nu_t);
+    const wf_real fw_kp = wf_fw_inv(wf_zibm_t(c, tp0, k + 1), ut0, inv_nu_t);
     /* damped mu_sgs at the node and its neighbours (a learned value, stored
      * negative under the closure, is not damped) */
-    const double mu_c = wf_cl_damped<CL>(a.dz(WF_D_MU, 0), fw);
-    const double mu_ip = wf_cl_damped<CL>(a.dv(WF_D_MU, 1, 0), fw_ip);
-    const double mu_im = wf_cl_damped<CL>(a.dv(WF_D_MU, -1, 0), fw_im);
-    const double mu_jp = wf_cl_damped<CL>(a.dv(WF_D_MU, 0, 1), fw_jp);
-    const double mu_jm = wf_cl_damped<CL>(a.dv(WF_D_MU, 0, -1), fw_jm);
-    const double mu_kp = wf_cl_damped<CL>(a.dz(WF_D_MU, 1), fw_kp);
-    const double mu_km = wf_cl_damped<CL>(a.dz(WF_D_MU, -1), fw_km);
-    const double S11 = a.dz(WF_D_S11, 0), S12 = a.dz(WF_D_S12, 0), S13 = a.dz(WF_D_S13, 0);
-    const double S22 = a.dz(WF_D_S22, 0), S23 = a.dz(WF_D_S23, 0), S33 = a.dz(WF_D_S33, 0);
-    const double mu_x = (mu_ip - mu_im) * c.inv2dx;
-    const double mu_y = (mu_jp - mu_jm) * c.inv2dy;
-    const double mu_z = (mu_kp - mu_km) * dzc;
-    const double S11_x = (a.dv(WF_D_S11, 1, 0) - a.dv(WF_D_S11, -1, 0)) * c.inv2dx;
-    const double S12_x = (a.dv(WF_D_S12, 1, 0) - a.dv(WF_D_S12, -1, 0)) * c.inv2dx;
-    const double S13_x = (a.dv(WF_D_S13, 1, 0) - a.dv(WF_D_S13, -1, 0)) * c.inv2dx;
-    const double S21_y = (a.dv(WF_D_S12, 0, 1) - a.dv(WF_D_S12, 0, -1)) * c.inv2dy;
-    const double S22_y = (a.dv(WF_D_S22, 0, 1) - a.dv(WF_D_S22, 0, -1)) * c.inv2dy;
-    const double S23_y = (a.dv(WF_D_S23, 0, 1) - a.dv(WF_D_S23, 0, -1)) * c.inv2dy;
-    const double dU_x = (a.dv(WF_D_DIVU, 1, 0) - a.dv(WF_D_DIVU, -1, 0)) * c.inv2dx;
-    const double dU_y = (a.dv(WF_D_DIVU, 0, 1) - a.dv(WF_D_DIVU, 0, -1)) * c.inv2dy;
-    const double S31_z = (a.dz(WF_D_S13, 1) - a.dz(WF_D_S13, -1)) * dzc;
-    const double S32_z = (a.dz(WF_D_S23, 1) - a.dz(WF_D_S23, -1)) * dzc;
-    const double S33_z = (a.dz(WF_D_S33, 1) - a.dz(WF_D_S33, -1)) * dzc;
-    const double dU_z = (a.dz(WF_D_DIVU, 1) - a.dz(WF_D_DIVU, -1)) * dzc;
-    const double div_S_x = S11_x + S21_y + S31_z;
-    const double div_S_y = S12_x + S22_y + S32_z;
-    const double div_S_z = S13_x + S23_y + S33_z;
-    const double sgs_x = -2 * (mu_x * S11 + mu_y * S12 + mu_z * S13 + mu_c * div_S_x) * inv_rho;
-    const double sgs_y = -2 * (mu_x * S12 + mu_y * S22 + mu_z * S23 + mu_c * div_S_y) * inv_rho;
-    const double sgs_z = -2 * (mu_x * S13 + mu_y * S23 + mu_z * S33 + mu_c * div_S_z) * inv_rho;
-    const double sgs_q = -(mu_x * Tx + mu_y * Ty + mu_z * Tz + mu_c * (Txx + Tyy + Tzz)) * inv_rho * c.inv_Pr;
-    const double third = 1.0 / 3.0;
+    const wf_real mu_c = wf_cl_damped<CL>(a.dz(WF_D_MU, 0), fw);
+    const wf_real mu_ip = wf_cl_damped<CL>(a.dv(WF_D_MU, 1, 0), fw_ip);
+    const wf_real mu_im = wf_cl_damped<CL>(a.dv(WF_D_MU, -1, 0), fw_im);
+    const wf_real mu_jp = wf_cl_damped<CL>(a.dv(WF_D_MU, 0, 1), fw_jp);
+    const wf_real mu_jm = wf_cl_damped<CL>(a.dv(WF_D_MU, 0, -1), fw_jm);
+    const wf_real mu_kp = wf_cl_damped<CL>(a.dz(WF_D_MU, 1), fw_kp);
+    const wf_real mu_km = wf_cl_damped<CL>(a.dz(WF_D_MU, -1), fw_km);
+    const wf_real S11 = a.dz(WF_D_S11, 0), S12 = a.dz(WF_D_S12, 0), S13 = a.dz(WF_D_S13, 0);
+    const wf_real S22 = a.dz(WF_D_S22, 0), S23 = a.dz(WF_D_S23, 0), S33 = a.dz(WF_D_S33, 0);
+    const wf_real mu_x = (mu_ip - mu_im) * c.inv2dx;
+    const wf_real mu_y = (mu_jp - mu_jm) * c.inv2dy;
+    const wf_real mu_z = (mu_kp - mu_km) * dzc;
+    const wf_real S11_x = (a.dv(WF_D_S11, 1, 0) - a.dv(WF_D_S11, -1, 0)) * c.inv2dx;
+    const wf_real S12_x = (a.dv(WF_D_S12, 1, 0) - a.dv(WF_D_S12, -1, 0)) * c.inv2dx;
+    const wf_real S13_x = (a.dv(WF_D_S13, 1, 0) - a.dv(WF_D_S13, -1, 0)) * c.inv2dx;
+    const wf_real S21_y = (a.dv(WF_D_S12, 0, 1) - a.dv(WF_D_S12, 0, -1)) * c.inv2dy;
+    const wf_real S22_y = (a.dv(WF_D_S22, 0, 1) - a.dv(WF_D_S22, 0, -1)) * c.inv2dy;
+    const wf_real S23_y = (a.dv(WF_D_S23, 0, 1) - a.dv(WF_D_S23, 0, -1)) * c.inv2dy;
+    const wf_real dU_x = (a.dv(WF_D_DIVU, 1, 0) - a.dv(WF_D_DIVU, -1, 0)) * c.inv2dx;
+    const wf_real dU_y = (a.dv(WF_D_DIVU, 0, 1) - a.dv(WF_D_DIVU, 0, -1)) * c.inv2dy;
+    const wf_real S31_z = (a.dz(WF_D_S13, 1) - a.dz(WF_D_S13, -1)) * dzc;
+    const wf_real S32_z = (a.dz(WF_D_S23, 1) - a.dz(WF_D_S23, -1)) * dzc;
+    const wf_real S33_z = (a.dz(WF_D_S33, 1) - a.dz(WF_D_S33, -1)) * dzc;
+    const wf_real dU_z = (a.dz(WF_D_DIVU, 1) - a.dz(WF_D_DIVU, -1)) * dzc;
+    const wf_real div_S_x = S11_x + S21_y + S31_z;
+    const wf_real div_S_y = S12_x + S22_y + S32_z;
+    const wf_real div_S_z = S13_x + S23_y + S33_z;
+    const wf_real sgs_x = -2 * (mu_x * S11 + mu_y * S12 + mu_z * S13 + mu_c * div_S_x) * inv_rho;
+    const wf_real sgs_y = -2 * (mu_x * S12 + mu_y * S22 + mu_z * S23 + mu_c * div_S_y) * inv_rho;
+    const wf_real sgs_z = -2 * (mu_x * S13 + mu_y * S23 + mu_z * S33 + mu_c * div_S_z) * inv_rho;
+    const wf_real sgs_q = -(mu_x * Tx + mu_y * Ty + mu_z * Tz + mu_c * (Txx + Tyy + Tzz)) * inv_rho * c.inv_Pr;
+    const wf_real third = WR(1.0) / WR(3.0);
     r[0] = u_RHS + (-sgs_x + dU_x * third);
     r[1] = v_RHS + (-sgs_y + dU_y * third);
     r[2] = w_RHS + (-sgs_z + dU_z * third);
     r[3] = T_RHS - sgs_q;
     /* fuel (src/cu/pde.cu:672-675): -Y_f K(T) Y where the column stores fuel */
-    r[4] = (k < nzy) ? -c.Y_f * K_T * a.Y() : 0.0;
+    r[4] = (k < nzy) ? -c.Y_f * K_T * a.Y() : WR(0.0);
 }
 
-/* RK combination of a node (src/cu/utils.cu:14-21, src/cu/solver.cu:14-39).
- * mode WF_MODE_PHI writes the stage derivative r; otherwise `out` receives
- * the next stage input (or y_{n+1}) and `acc` the running combination. */
-__device__ __forceinline__ void wf_rk_write(int mode, double a, unsigned q, const double r[5], int nf,
-                                            const WfState &Yn, const WfState &acc, const WfState &out) {
-    double *o[5] = {out.u, out.v, out.w, out.T, out.Y};
-    const double *yn[5] = {Yn.u, Yn.v, Yn.w, Yn.T, Yn.Y};
-    double *ac[5] = {acc.u, acc.v, acc.w, acc.T, acc.Y};
-#pragma unroll
-    for (int f = 0; f < 5; f++) {
-        if (f >= nf) break;
-        double kf = r[f];
-        double res;
-        switch (mode) {
-            case WF_MODE_PHI:
-                res = kf;
-                break;
-            case WF_MODE_FIRST: /* acc = k1; next = y_n + a k1 */
-                ac[f][q] = kf;
-                res = yn[f][q] + a * kf;
-                break;
-            case WF_MODE_MID: /* acc += 2 k; next = y_n + a k */
-                ac[f][q] = ac[f][q] + 2 * kf;
-                res = yn[f][q] + a * kf;
-                break;
-            case WF_MODE_LAST: /* y_{n+1} = y_n + a (acc + k) */
-                res = yn[f][q] + a * (ac[f][q] + kf);
-                break;
-            default: /* WF_MODE_EULER: y_{n+1} = y_n + a k */
-                res = yn[f][q] + a * kf;
-                break;
-        }
-        o[f][q] = res;
+/* RK combination of one value at node q (src/cu/utils.cu:14-21,
+ * src/cu/solver.cu:14-39) in type V: wf_real for u, v, w, T; double for Y,
+ * whose stage derivative is formed in wf_real and accumulated in FP64.  yn and
+ * acv are the operands (y_n and the running combination); mode WF_MODE_PHI
+ * writes the stage derivative itself. */
+template <typename V>
+__device__ __forceinline__ void wf_rk_one(int mode, double a, unsigned q, V kf, V yn, V acv, V *ac, V *o) {
+    V res;
+    switch (mode) {
+        case WF_MODE_PHI:
+            res = kf;
+            break;
+        case WF_MODE_FIRST: /* acc = k1; next = y_n + a k1 */
+            ac[q] = kf;
+            res = yn + (V)a * kf;
+            break;
+        case WF_MODE_MID: /* acc += 2 k; next = y_n + a k */
+            ac[q] = acv + 2 * kf;
+            res = yn + (V)a * kf;
+            break;
+        case WF_MODE_LAST: /* y_{n+1} = y_n + a (acc + k) */
+            res = yn + (V)a * (acv + kf);
+            break;
+        default: /* WF_MODE_EULER: y_{n+1} = y_n + a k */
+            res = yn + (V)a * kf;
+            break;
     }
+    o[q] = res;
 }
 
 /* Operands of the RK combination of node q (y_n, and the running
- * combination in modes MID and LAST), loaded before the stage stores anything:
- * `out` never aliases y_n or the accumulator, so issuing them early lets their
- * latency hide under the RHS evaluation instead of trailing it, one field at
- * a time.  Y keeps Nz_Y_max planes: above them the Y operands are read from
- * the node's k = 0 entry (or from T when there is no Y) and not used. */
+ * combination in modes MID and LAST).  Y keeps max(Nz_Y_max, 1) planes: above
+ * them the Y operands are read from the node's k = 0 entry and not used. */
 struct WfRkIn {
-    double yn[5], ac[5];
+    wf_real yn[4], ac[4];
+    double ynY, acY;
 };
 __device__ __forceinline__ void wf_rk_load(const WfConst &c, int mode, unsigned q, unsigned col, int k,
                                            const WfState &Yn, const WfState &acc, WfRkIn &in) {
@@ -521,57 +516,44 @@ __device__ __forceinline__ void wf_rk_load(const WfConst &c, int mode, unsigned 
         in.yn[1] = wf_ld(Yn.v, q);
         in.yn[2] = wf_ld(Yn.w, q);
         in.yn[3] = wf_ld(Yn.T, q);
-        in.yn[4] = wf_ld((c.Nzm > 0) ? Yn.Y : Yn.T, qy);
+        in.ynY = wf_ld(Yn.Y, qy);
     }
     if (mode == WF_MODE_MID || mode == WF_MODE_LAST) {
         in.ac[0] = wf_ld(acc.u, q);
         in.ac[1] = wf_ld(acc.v, q);
         in.ac[2] = wf_ld(acc.w, q);
         in.ac[3] = wf_ld(acc.T, q);
-        in.ac[4] = wf_ld((c.Nzm > 0) ? acc.Y : acc.T, qy);
+        in.acY = wf_ld(acc.Y, qy);
     }
 }
 
-/* The RK combination of wf_rk_write from operands loaded by wf_rk_load. */
-__device__ __forceinline__ void wf_rk_store(int mode, double a, unsigned q, const double r[5], int nf,
+/* The RK combination from operands loaded by wf_rk_load: fields u, v, w, T
+ * and, when withY, Y. */
+__device__ __forceinline__ void wf_rk_store(int mode, double a, unsigned q, const wf_real r[5], bool withY,
                                             const WfRkIn &in, const WfState &acc, const WfState &out) {
-    double *o[5] = {out.u, out.v, out.w, out.T, out.Y};
-    double *ac[5] = {acc.u, acc.v, acc.w, acc.T, acc.Y};
+    wf_real *o[4] = {out.u, out.v, out.w, out.T};
+    wf_real *ac[4] = {acc.u, acc.v, acc.w, acc.T};
 #pragma unroll
-    for (int f = 0; f < 5; f++) {
-        if (f >= nf) break;
-        const double kf = r[f];
-        double res;
-        switch (mode) {
-            case WF_MODE_PHI:
-                res = kf;
-                break;
-            case WF_MODE_FIRST:
-                ac[f][q] = kf;
-                res = in.yn[f] + a * kf;
-                break;
-            case WF_MODE_MID:
-                ac[f][q] = in.ac[f] + 2 * kf;
-                res = in.yn[f] + a * kf;
-                break;
-            case WF_MODE_LAST:
-                res = in.yn[f] + a * (in.ac[f] + kf);
-                break;
-            default:
-                res = in.yn[f] + a * kf;
-                break;
-        }
-        o[f][q] = res;
-    }
+    for (int f = 0; f < 4; f++) wf_rk_one<wf_real>(mode, a, q, r[f], in.yn[f], in.ac[f], ac[f], o[f]);
+    if (withY) wf_rk_one<double>(mode, a, q, (double)r[4], in.ynY, in.acY, acc.Y, out.Y);
+}
+
+/* The same with the operands read here (column completion). */
+__device__ __forceinline__ void wf_rk_write(const WfConst &c, int mode, double a, unsigned q, int k,
+                                            const wf_real r[5], bool withY, const WfState &Yn, const WfState &acc,
+                                            const WfState &out) {
+    WfRkIn in;
+    wf_rk_load(c, mode, q, q % (unsigned)c.plane, k, Yn, acc, in);
+    wf_rk_store(mode, a, q, r, withY, in, acc, out);
 }
 
 
 /* Raw RHS of the nodes cut+1, cut+2, Nz-2 and Nz-3, kept per column for the
  * cut and top nodes (k_stage_columns). */
 __device__ __forceinline__ void wf_keep_raw(const WfConst &c, const WfDerived &D, int col, int k, int cut,
-                                            const double r[5]) {
+                                            const wf_real r[5]) {
     const unsigned pl = (unsigned)c.plane;
-    double *keep = (k == cut + 1) ? D.col : (k == cut + 2) ? D.col + 5 * pl : nullptr;
+    wf_real *keep = (k == cut + 1) ? D.col : (k == cut + 2) ? D.col + 5 * pl : nullptr;
     if (keep) {
 #pragma unroll
         for (int f = 0; f < 5; f++) keep[f * pl + col] = r[f];
@@ -597,19 +579,19 @@ __device__ __forceinline__ void wf_stage_dup_node(const WfConst &c, const WfStat
     const long q = wf_q(c, i, j, k);
     WfRkIn rk;
     wf_rk_load(c, mode, (unsigned)q, (unsigned)col, k, Yn, acc, rk);
-    double r[5];
+    wf_real r[5];
     if (k < cut) {
         r[0] = c.u_dead;
         r[1] = c.v_dead;
         r[2] = c.w_dead;
         r[3] = c.T_dead;
-        r[4] = (k < nzy) ? c.Y_dead : 0.0;
+        r[4] = (k < nzy) ? (wf_real)c.Y_dead : WR(0.0);
     } else {
         const WfGlobalAcc acc_g{c, X, D, i, j, k};
         wf_rhs_node<CL>(c, acc_g, k, nzy, r);
         wf_keep_raw(c, D, col, k, cut, r);
     }
-    wf_rk_store(mode, a, (unsigned)q, r, (k < c.Nzm) ? 5 : 4, rk, acc, out);
+    wf_rk_store(mode, a, (unsigned)q, r, k < c.Nzm, rk, acc, out);
 }
 
 /* One stage on the nodes below the top plane: dead nodes (k < cut) take the
@@ -653,12 +635,12 @@ __global__ __launch_bounds__(WF_NT_STAGE) WF_STAGE_ATTR void k_stage(WfConst c, 
     {
         constexpr int PS = (WF_SW * WF_SHH + WF_NT_STAGE - 1) / WF_NT_STAGE;
         constexpr int PD = (WF_DW * WF_DHH + WF_NT_STAGE - 1) / WF_NT_STAGE;
-        double vs[4][PS], vd[7][PD], vu[PD], vt[PD];
+        wf_real vs[4][PS], vd[7][PD], vu[PD], vt[PD];
         wf_tile_load<WF_SW, WF_SHH, WF_SH>(vs[0], X.u, c, i0, j0, k, tid);
         wf_tile_load<WF_SW, WF_SHH, WF_SH>(vs[1], X.v, c, i0, j0, k, tid);
         wf_tile_load<WF_SW, WF_SHH, WF_SH>(vs[2], X.w, c, i0, j0, k, tid);
         wf_tile_load<WF_SW, WF_SHH, WF_SH>(vs[3], X.T, c, i0, j0, k, tid);
-        const double *dsrc[7] = {D.mu, D.s11, D.s12, D.s13, D.s22, D.s23, D.divU};
+        const wf_real *dsrc[7] = {D.mu, D.s11, D.s12, D.s13, D.s22, D.s23, D.divU};
 #pragma unroll
         for (int f = 0; f < 7; f++) wf_tile_load<WF_DW, WF_DHH, WF_DH>(vd[f], dsrc[f], c, i0, j0, k, tid);
         wf_tile_load<WF_DW, WF_DHH, WF_DH>(vu, D.utau, c, i0, j0, 0, tid);
@@ -675,7 +657,7 @@ __global__ __launch_bounds__(WF_NT_STAGE) WF_STAGE_ATTR void k_stage(WfConst c, 
     {
         const unsigned km2 = (k >= 2) ? q - 2 * P : q, kp2 = (k <= c.Nz - 3) ? q + 2 * P : q;
         const unsigned km1 = (k >= 1) ? q - P : q, kp1 = q + P;
-        const double *sf[4] = {X.u, X.v, X.w, X.T};
+        const wf_real *sf[4] = {X.u, X.v, X.w, X.T};
 #pragma unroll
         for (int f = 0; f < 4; f++) {
             V.s[f][0] = wf_ld(sf[f], km2);
@@ -702,8 +684,8 @@ __global__ __launch_bounds__(WF_NT_STAGE) WF_STAGE_ATTR void k_stage(WfConst c, 
         V.s11 = wf_ld(D.s11, q);
         V.s12 = wf_ld(D.s12, q);
         V.s22 = wf_ld(D.s22, q);
-        /* Y stores Nz_Y_max planes (possibly none): other planes read a T node */
-        V.Y = wf_ld((c.Nzm > 0) ? X.Y : X.T, (k < c.Nzm) ? q : (unsigned)col);
+        /* Y stores max(Nz_Y_max, 1) planes: other planes read the k = 0 entry */
+        V.Y = (wf_real)wf_ld(X.Y, (k < c.Nzm) ? q : (unsigned)col);
         V.utau = wf_ld(D.utau, (unsigned)col);
         V.topo = wf_ld(c.topo, (unsigned)col);
     }
@@ -715,19 +697,19 @@ __global__ __launch_bounds__(WF_NT_STAGE) WF_STAGE_ATTR void k_stage(WfConst c, 
      * in their own row (column), which the tiles hold as row (column) 0:
      * k_stage_dup evaluates them */
     if (!inside || k == cut || i == c.Nx - 1 || j == c.Ny - 1) return;
-    double r[5];
+    wf_real r[5];
     if (k < cut) {
         r[0] = c.u_dead;
         r[1] = c.v_dead;
         r[2] = c.w_dead;
         r[3] = c.T_dead;
-        r[4] = (k < nzy) ? c.Y_dead : 0.0;
+        r[4] = (k < nzy) ? (wf_real)c.Y_dead : WR(0.0);
     } else {
         const WfTileAcc acc_t{L, V, (ty + WF_SH) * WF_SW + (tx + WF_SH), (ty + WF_DH) * WF_DW + (tx + WF_DH)};
         wf_rhs_node<CL>(c, acc_t, k, nzy, r);
         wf_keep_raw(c, D, col, k, cut, r);
     }
-    wf_rk_store(mode, a, q, r, (k < c.Nzm) ? 5 : 4, rk, acc, out);
+    wf_rk_store(mode, a, q, r, k < c.Nzm, rk, acc, out);
 }
 
 /* The duplicate-plane nodes of k_stage in a launch of their own: one thread
@@ -742,10 +724,10 @@ __global__ __launch_bounds__(256) void k_stage_dup(WfConst c, WfState X, WfDeriv
 
 /* Extrapolation giving a zero one-sided z-derivative at the bottom of the fluid
  * (src/cu/pde.cu:62) and at the top (src/cu/pde.cu:84-87). */
-__device__ __forceinline__ double wf_extrap_bottom(double f1, double f2, double h0, double h1) {
+__device__ __forceinline__ wf_real wf_extrap_bottom(wf_real f1, wf_real f2, wf_real h0, wf_real h1) {
     return ((h0 + h1) * (h0 + h1) * f1 - h0 * h0 * f2) / (h1 * (2 * h0 + h1));
 }
-__device__ __forceinline__ double wf_extrap_top(double fm1, double fm2, double hm2, double hm1) {
+__device__ __forceinline__ wf_real wf_extrap_top(wf_real fm1, wf_real fm2, wf_real hm2, wf_real hm1) {
     return ((hm2 + hm1) * (hm2 + hm1) * fm1 - hm1 * hm1 * fm2) / (hm2 * (hm2 + 2 * hm1));
 }
 
@@ -760,48 +742,50 @@ __global__ void k_stage_columns(WfConst c, WfState X, WfDerived D, WfState Yn, W
     const unsigned pl = (unsigned)c.plane;
     const int col = i + c.Nx * j;
     const int cut = c.cut[col], nzy = c.nzy[col];
-    const double *z = c.z;
-    const double *b1 = D.col, *b2 = D.col + 5 * pl, *t1 = D.col + 10 * pl, *t2 = D.col + 15 * pl;
-    double r[5];
+    const wf_real *z = c.z;
+    const wf_real *b1 = D.col, *b2 = D.col + 5 * pl, *t1 = D.col + 10 * pl, *t2 = D.col + 15 * pl;
+    wf_real r[5];
     {
         const int k = cut;
-        const double h0 = z[k + 1] - z[k], h1 = z[k + 2] - z[k + 1];
-        r[0] = 0.0;
-        r[1] = 0.0;
-        r[2] = 0.0;
+        const wf_real h0 = z[k + 1] - z[k], h1 = z[k + 2] - z[k + 1];
+        r[0] = WR(0.0);
+        r[1] = WR(0.0);
+        r[2] = WR(0.0);
         r[3] = wf_extrap_bottom(b1[3 * pl + col], b2[3 * pl + col], h0, h1);
-        r[4] = (k < nzy) ? wf_extrap_bottom(b1[4 * pl + col], b2[4 * pl + col], h0, h1) : 0.0;
-        wf_rk_write(mode, a, (unsigned)col + pl * (unsigned)k, r, (k < c.Nzm) ? 5 : 4, Yn, acc, out);
+        r[4] = (k < nzy) ? wf_extrap_bottom(b1[4 * pl + col], b2[4 * pl + col], h0, h1) : WR(0.0);
+        wf_rk_write(c, mode, a, (unsigned)col + pl * (unsigned)k, k, r, k < c.Nzm, Yn, acc, out);
     }
     {
         const int k = c.Nz - 1;
         const unsigned q = (unsigned)col + pl * (unsigned)k;
-        const double hm1 = z[k] - z[k - 1], hm2 = z[k - 1] - z[k - 2];
+        const wf_real hm1 = z[k] - z[k - 1], hm2 = z[k - 1] - z[k - 2];
 #pragma unroll
         for (int f = 0; f < 4; f++) r[f] = wf_extrap_top(t1[f * pl + col], t2[f * pl + col], hm2, hm1);
-        r[4] = (k < nzy) ? -c.Y_f * wf_rate(c, X.T[q]) * X.Y[q] : 0.0;
-        wf_rk_write(mode, a, q, r, (k < c.Nzm) ? 5 : 4, Yn, acc, out);
+        r[4] = (k < nzy) ? -c.Y_f * wf_rate(c, X.T[q]) * (wf_real)X.Y[q] : WR(0.0);
+        wf_rk_write(c, mode, a, q, k, r, k < c.Nzm, Yn, acc, out);
     }
 }
 
 /* --------------------------------------------------------------- layout ---- */
 
-/* Reference layout (k fastest) -> internal layout (i fastest) for `nfields`
- * consecutive scalar fields of nz planes each (nz = Nz, or Nz_Y_max for Y). */
-__global__ void k_to_internal(const double *src, double *dst, int Nx, int Ny, int nz) {
+/* Reference layout (k fastest, FP64) -> internal layout (i fastest, type D)
+ * of one scalar field of nz planes (nz = Nz, or Nz_Y_max for Y). */
+template <typename D>
+__global__ void k_to_internal(const double *src, D *dst, int Nx, int Ny, int nz) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     int j = blockIdx.y;
     int k = blockIdx.z;
     if (i >= Nx) return;
-    dst[(long)i + (long)Nx * ((long)j + (long)Ny * k)] = src[(long)k + (long)nz * ((long)j + (long)Ny * i)];
+    dst[(long)i + (long)Nx * ((long)j + (long)Ny * k)] = (D)src[(long)k + (long)nz * ((long)j + (long)Ny * i)];
 }
 
-__global__ void k_to_reference(const double *src, double *dst, int Nx, int Ny, int nz) {
+template <typename D>
+__global__ void k_to_reference(const D *src, double *dst, int Nx, int Ny, int nz) {
     int k = blockIdx.x * blockDim.x + threadIdx.x;
     int j = blockIdx.y;
     int i = blockIdx.z;
     if (k >= nz) return;
-    dst[(long)k + (long)nz * ((long)j + (long)Ny * i)] = src[(long)i + (long)Nx * ((long)j + (long)Ny * k)];
+    dst[(long)k + (long)nz * ((long)j + (long)Ny * i)] = (double)src[(long)i + (long)Nx * ((long)j + (long)Ny * k)];
 }
 
 /* ------------------------------------------------------------- launchers ---- */
@@ -845,12 +829,12 @@ hipError_t wf_launch_stage(const WfConst &c, const WfState &X, const WfDerived &
 
 /* The network alone on n rows of raw inputs (row-major n x 12): nu_t per row
  * and whether the row is in the training range with a finite output. */
-__global__ __launch_bounds__(256) void k_closure_eval(WfClosure cl, const double *feat, long n, double *nu,
+__global__ __launch_bounds__(256) void k_closure_eval(WfClosure cl, const wf_real *feat, long n, wf_real *nu,
                                                       int *learned) {
     __shared__ WfClosureLds lds[4];
     const long row = (long)blockIdx.x * 256 + threadIdx.x;
     const long rc = (row < n) ? row : n - 1;
-    double f[WF_CL_NIN];
+    wf_real f[WF_CL_NIN];
 #pragma unroll
     for (int i = 0; i < WF_CL_NIN; i++) f[i] = feat[rc * WF_CL_NIN + i];
     __bf16 x[WF_CL_NIN];
@@ -858,27 +842,38 @@ __global__ __launch_bounds__(256) void k_closure_eval(WfClosure cl, const double
     const float z = wf_cl_forward(cl, lds[threadIdx.x >> 6], x);
     if (row >= n) return;
     const float v = cl.prm[WF_CL_SCALE] * wf_cl_softplus(z);
-    nu[row] = (double)v;
+    nu[row] = (wf_real)v;
     learned[row] = (in_range && v >= 0.0f && v < 3.0e38f) ? 1 : 0;
 }
 
-hipError_t wf_launch_closure_eval(const WfClosure &cl, const double *feat, long n, double *nu, int *learned,
+hipError_t wf_launch_closure_eval(const WfClosure &cl, const wf_real *feat, long n, wf_real *nu, int *learned,
                                   hipStream_t s) {
     if (n <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_closure_eval, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, cl, feat, n, nu, learned);
     return hipGetLastError();
 }
 
-hipError_t wf_launch_to_internal(const double *src, double *dst, int Nx, int Ny, int nz, hipStream_t s) {
+template <typename D>
+hipError_t wf_launch_to_internal(const double *src, D *dst, int Nx, int Ny, int nz, hipStream_t s) {
     if (nz <= 0) return hipSuccess;
     dim3 blk(64, 1, 1), grd((Nx + 63) / 64, Ny, nz);
-    hipLaunchKernelGGL(k_to_internal, grd, blk, 0, s, src, dst, Nx, Ny, nz);
+    hipLaunchKernelGGL(k_to_internal<D>, grd, blk, 0, s, src, dst, Nx, Ny, nz);
     return hipGetLastError();
 }
 
-hipError_t wf_launch_to_reference(const double *src, double *dst, int Nx, int Ny, int nz, hipStream_t s) {
+template <typename D>
+hipError_t wf_launch_to_reference(const D *src, double *dst, int Nx, int Ny, int nz, hipStream_t s) {
     if (nz <= 0) return hipSuccess;
     dim3 blk(64, 1, 1), grd((nz + 63) / 64, Ny, Nx);
-    hipLaunchKernelGGL(k_to_reference, grd, blk, 0, s, src, dst, Nx, Ny, nz);
+    hipLaunchKernelGGL(k_to_reference<D>, grd, blk, 0, s, src, dst, Nx, Ny, nz);
     return hipGetLastError();
 }
+
+template hipError_t wf_launch_to_internal<double>(const double *, double *, int, int, int, hipStream_t);
+template hipError_t wf_launch_to_reference<double>(const double *, double *, int, int, int, hipStream_t);
+#if WF_MIXED
+template hipError_t wf_launch_to_internal<float>(const double *, float *, int, int, int, hipStream_t);
+template hipError_t wf_launch_to_reference<float>(const float *, double *, int, int, int, hipStream_t);
+#endif
+
+}  // namespace WF_NS
