@@ -353,15 +353,24 @@ KERNEL_CLASSES = (("k_derived", "derived"), ("k_stage", "stage"), ("k_rhs_fx", "
 
 def _kernel_class(name):
     """bench kernel class of a rocprofv3 kernel name (None if not timed by
-    class); the top-plane transform k_rhs_fx<n, 3> runs once per run, not per
-    solve, and is left out."""
+    class): the FP64 kernels (namespace wf64) by their class, the mixed form's
+    (wf32) as "mixed_<class>", the closure's derived pass as
+    "<...>derived_closure".  The top-plane transform k_rhs_fx<n, 3> runs once
+    per run, not per solve, and is left out."""
     short = _short_name(name)
+    prefix_ns = ""
+    if short.startswith("wf32::"):
+        prefix_ns, short = "mixed_", short[6:]
+    elif short.startswith("wf64::"):
+        short = short[6:]
     if short.startswith("k_rhs_fx") and short.replace(" ", "").endswith(",3>"):
         return None
     base = short.split("<")[0]
     for prefix, cls in KERNEL_CLASSES:
         if base == prefix or (prefix == "k_stage" and base in ("k_stage_columns", "k_stage_dup")):
-            return cls
+            if cls == "derived" and short.startswith("k_derived<true>"):
+                cls = "derived_closure"
+            return prefix_ns + cls
     return None
 
 
@@ -412,7 +421,7 @@ def traffic_json(pmc_dir, cal_dir, copy_bytes):
             # k_stage_dup and k_stage_columns are companion launches of
             # k_stage: their bytes add to its launch, they do not count
             a[counter][0] += v * 1024.0 * scales[counter]
-            if not _short_name(name).startswith(("k_stage_columns", "k_stage_dup")):
+            if not _short_name(name).split("::")[-1].startswith(("k_stage_columns", "k_stage_dup")):
                 a[counter][1] += 1
     kernels = {}
     for cls, a in sorted(acc.items()):

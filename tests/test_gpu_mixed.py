@@ -6,7 +6,8 @@ the convergence reductions run in FP64.  The FP64 oracle (or the FP64 GPU
 path, which matches it to 1e-10) is the reference:
   - one right-hand-side evaluation: <= 5e-6 relative L2 per field (FP32
     stencil arithmetic, a few units of 2^-24 amplified by the differences);
-  - one pressure solve: <= 5e-6 relative L2;
+  - one pressure solve: <= 5e-5 relative L2 (the FP32 iteration stops once
+    max|dp| <= 3e-5 max|p|, its accuracy floor; the FP64 one at 1e-10);
   - 20 steps of the small cases and 400 steps of the burning terrain case:
     <= 1e-4 per field (measured <= 6e-6), the bound the mixed form is held to;
   - Y after 20 steps: <= 1e-8, below FP32's own rounding of Y (~3e-8), which
@@ -63,7 +64,7 @@ def test_mixed_pressure_close_to_fp64(case):
     pm, em, im = api.Solver(case, "mixed").pressure(y1, y, p)
     pd, ed, idd = api.Solver(case, "fp64").pressure(y1, y, p)
     print("mixed vs fp64 %.3e; mixed err %.2e after %d, fp64 %.2e after %d" % (rel_l2(pm, pd), em, im, ed, idd))
-    assert rel_l2(pm, pd) <= 5e-6
+    assert rel_l2(pm, pd) <= 5e-5
 
 
 def test_mixed_steps_close_to_fp64(case):

@@ -115,7 +115,7 @@ struct WfSpectral {
     wf_real *A;     /* Nx*Ny*Nz: p-independent part of the Poisson RHS */
     const double *UB;    /* [chunk][hx][ldj]: Thomas pivot u_{k0-1} entering each z-chunk (chunk >= 1), FP64 */
     int thomas_lc;       /* z-levels per chunk of the tridiagonal solve */
-    double *pmax;        /* npmax partial maxima of |p_new - p_old| */
+    double *pmax;        /* npmax partial maxima of |p_new - p_old| (mixed form: then npmax of |p_new|) */
     const wf_real *kx2;  /* hx: kx^2 of the retained x-wavenumbers */
     const wf_real *ky2;  /* ny */
     const wf_real *ta;   /* nz-1: sub-diagonal a_k (k-only) */
@@ -151,7 +151,7 @@ hipError_t wf_launch_iter_reset(WfIterCtl *ctl, hipStream_t s);
 hipError_t wf_launch_thomas(const WfSpectral &sp, hipStream_t s);
 hipError_t wf_launch_pivots(const WfSpectral &sp, double *UB, hipStream_t s);
 hipError_t wf_launch_inverse_fx(const WfConst &c, const WfSpectral &sp, wf_real *p, hipStream_t s);
-hipError_t wf_launch_iter_check(const WfSpectral &sp, double tol, int m, int max_iter, hipStream_t s);
+hipError_t wf_launch_iter_check(const WfSpectral &sp, double tol, double rtol, int m, int max_iter, hipStream_t s);
 
 /* Temperature source applied at the end of a step (src/cu/functions.cu:325-403). */
 struct WfSource {

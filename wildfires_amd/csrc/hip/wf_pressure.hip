@@ -183,39 +183,53 @@ __global__ void k_iter_reset(WfIterCtl *ctl) {
     ctl->err = 0.0;
 }
 
-/* After solve m: err = max|p_new - p_old| (the maximum of the inverse
- * x-transform's per-workgroup maxima), stop once err <= tol or after max_iter
- * solves (src/cu/pressure.cu:564-574); iters is the count the reference
- * reports (the index of the converging solve, else max_iter).  The flag and
- * the solve count are also written to mapped host memory so the host can stop
- * queueing solves without a round trip.  (A separate one-workgroup launch: on
- * a multi-XCD device a grid-wide reduction inside the inverse transform would
- * need a device-scope release, an L2 write-back, in every workgroup.) */
-__global__ __launch_bounds__(WF_NT) void k_iter_check(WfSpectral sp, double tol, int m, int max_iter) {
-    WfIterCtl *ctl = sp.ctl;
-    if (ctl->done) return;
-    /* 16 independent loads per thread and pass, so one pass costs one memory
-     * round trip (a plain strided loop waits for each load in turn) */
+/* Maximum of the n partial maxima a[] (>= 0) over the workgroup, into *out
+ * (16 independent loads per thread and pass, so one pass costs one memory
+ * round trip; a plain strided loop waits for each load in turn). */
+__device__ __forceinline__ void wf_reduce_max(const double *a, int n, double *out) {
     constexpr int U = 16;
     double v = 0.0;
-    for (int t0 = 0; t0 < sp.npmax; t0 += U * WF_NT) {
+    for (int t0 = 0; t0 < n; t0 += U * WF_NT) {
         double x[U];
 #pragma unroll
         for (int u = 0; u < U; u++) {
             const int t = t0 + u * WF_NT + (int)threadIdx.x;
-            x[u] = (t < sp.npmax) ? sp.pmax[t] : 0.0;
+            x[u] = (t < n) ? a[t] : 0.0;
         }
 #pragma unroll
         for (int u = 0; u < U; u++) v = (x[u] > v) ? x[u] : v;
     }
-    __shared__ double err;
-    wf_block_max_store(v, &err);
+    wf_block_max_store(v, out);
     __syncthreads();
+}
+
+/* After solve m: err = max|p_new - p_old| (the maximum of the inverse
+ * x-transform's per-workgroup maxima), stop once err <= tol or after max_iter
+ * solves (src/cu/pressure.cu:564-574); iters is the count the reference
+ * reports (the index of the converging solve, else max_iter).  The mixed form
+ * also stops once err <= rtol max|p_new|: below that the FP32 iteration no
+ * longer moves p toward the solution (its accuracy floor).  The flag and the
+ * solve count are also written to mapped host memory so the host can stop
+ * queueing solves without a round trip.  (A separate one-workgroup launch: on
+ * a multi-XCD device a grid-wide reduction inside the inverse transform would
+ * need a device-scope release, an L2 write-back, in every workgroup.) */
+__global__ __launch_bounds__(WF_NT) void k_iter_check(WfSpectral sp, double tol, double rtol, int m, int max_iter) {
+    WfIterCtl *ctl = sp.ctl;
+    if (ctl->done) return;
+    __shared__ double err, pabs;
+    wf_reduce_max(sp.pmax, sp.npmax, &err);
+#if WF_MIXED
+    wf_reduce_max(sp.pmax + sp.npmax, sp.npmax, &pabs);
+#endif
     if (threadIdx.x != 0) return;
+    double tol_eff = tol;
+#if WF_MIXED
+    tol_eff = (rtol * pabs > tol_eff) ? rtol * pabs : tol_eff;
+#endif
     ctl->err = err;
     ctl->solves = m + 1;
     int done;
-    if (err <= tol) {
+    if (err <= tol_eff) {
         done = 1;
         ctl->iters = m;
     } else {
@@ -750,10 +764,14 @@ __global__ __launch_bounds__(64 * WF_THOMAS_MAX_CHUNKS) __attribute__((amdgpu_wa
 /* new value pv of node (i, j, k) and of its periodic duplicates, with the
  * running max |p_new - p_old| */
 __device__ __forceinline__ void wf_put_p_old(const WfConst &c, wf_real *p, int i, int j, int k, wf_real pv, wf_real pold,
-                                             double &dmax) {
+                                             double &dmax, double &pabs) {
     const long q = wf_q(c, i, j, k);
     const double d = (double)fabs(pv - pold);
     dmax = (d > dmax) ? d : dmax;
+    if constexpr (WF_MIXED) {
+        const double a = (double)fabs(pv);
+        pabs = (a > pabs) ? a : pabs;
+    }
     p[q] = pv;
     if (i == 0) { /* duplicate column i = Nx-1 */
         const long qd = wf_q(c, c.Nx - 1, j, k);
@@ -774,8 +792,9 @@ __device__ __forceinline__ void wf_put_p_old(const WfConst &c, wf_real *p, int i
         }
     }
 }
-__device__ __forceinline__ void wf_put_p(const WfConst &c, wf_real *p, int i, int j, int k, wf_real pv, double &dmax) {
-    wf_put_p_old(c, p, i, j, k, pv, p[wf_q(c, i, j, k)], dmax);
+__device__ __forceinline__ void wf_put_p(const WfConst &c, wf_real *p, int i, int j, int k, wf_real pv, double &dmax,
+                                         double &pabs) {
+    wf_put_p_old(c, p, i, j, k, pv, p[wf_q(c, i, j, k)], dmax, pabs);
 }
 
 /* Compile-time plan of P5: every global load (half spectra, the old p of the
@@ -787,7 +806,7 @@ __device__ __forceinline__ void wf_put_p(const WfConst &c, wf_real *p, int i, in
  * the norm, is the one of wf_put_p. */
 template <int NX>
 __device__ __forceinline__ void wf_inv_fx_planned(const WfConst &c, const WfSpectral &sp, wf_real *p, wfc *smem,
-                                                  int g, int k, double &dmax) {
+                                                  int g, int k, double &dmax, double &pabs) {
     constexpr int PAIRS = wf_pairs_for(NX);
     constexpr int PER = (PAIRS * NX + WF_NT - 1) / WF_NT;
     constexpr int PERY = (NX + 1 + WF_NT - 1) / WF_NT; /* row j = Ny-1: i = 0 .. Nx-1 */
@@ -870,12 +889,14 @@ __device__ __forceinline__ void wf_inv_fx_planned(const WfConst &c, const WfSpec
                 const wf_real pv = v.re * inv_scale;
                 const double d = (double)fabs(pv - pa[pi]);
                 dmax = (d > dmax) ? d : dmax;
+                if constexpr (WF_MIXED) pabs = ((double)fabs(pv) > pabs) ? (double)fabs(pv) : pabs;
                 p[wf_q(c, i, ja, k)] = pv;
             }
             if (jb < ny) {
                 const wf_real pv = v.im * inv_scale;
                 const double d = (double)fabs(pv - pb[pi]);
                 dmax = (d > dmax) ? d : dmax;
+                if constexpr (WF_MIXED) pabs = ((double)fabs(pv) > pabs) ? (double)fabs(pv) : pabs;
                 p[wf_q(c, i, jb, k)] = pv;
             }
         }
@@ -911,7 +932,7 @@ __global__ __launch_bounds__(WF_NT) void k_inv_fx(WfConst c, WfSpectral sp, wf_r
     const int nx = (NX > 0) ? NX : sp.nx;
     const int pairs = sp.fx_pairs;
     const int ny = sp.ny, hx = sp.hx, ldj = sp.ldj;
-    double dmax = 0.0;
+    double dmax = 0.0, pabs = 0.0;
     if (blockIdx.x == gridDim.x - 1) {
         const int kt = c.Nz - 1;
         for (int t = threadIdx.x; t < c.Ny + c.Nx - 1; t += blockDim.x) {
@@ -929,12 +950,16 @@ __global__ __launch_bounds__(WF_NT) void k_inv_fx(WfConst c, WfSpectral sp, wf_r
             p[qd] = v;
         }
         wf_block_max_store(dmax, &sp.pmax[blockIdx.x]);
+        if constexpr (WF_MIXED) {
+            __syncthreads();
+            wf_block_max_store(pabs, &sp.pmax[sp.npmax + blockIdx.x]);
+        }
         return;
     }
     int g, k;
     wf_tile_decode(sp.ngroups, sp.nz, &g, &k);
     if constexpr (NX > 0) {
-        wf_inv_fx_planned<NX>(c, sp, p, smem, g, k, dmax);
+        wf_inv_fx_planned<NX>(c, sp, p, smem, g, k, dmax, pabs);
     } else {
         const int j0 = 2 * g * pairs;
         const wf_real2 *src = sp.F + (long)k * hx * ldj;
@@ -971,11 +996,15 @@ __global__ __launch_bounds__(WF_NT) void k_inv_fx(WfConst c, WfSpectral sp, wf_r
             const int s = t / (Nx - 1), i = t - s * (Nx - 1);
             const int ja = j0 + 2 * s, jb = ja + 1;
             if (ja >= ny) continue;
-            wf_put_p(c, p, i, ja, k, res[s * nx + i].re * inv_scale, dmax);
-            if (jb < ny) wf_put_p(c, p, i, jb, k, res[s * nx + i].im * inv_scale, dmax);
+            wf_put_p(c, p, i, ja, k, res[s * nx + i].re * inv_scale, dmax, pabs);
+            if (jb < ny) wf_put_p(c, p, i, jb, k, res[s * nx + i].im * inv_scale, dmax, pabs);
         }
     }
     wf_block_max_store(dmax, &sp.pmax[blockIdx.x]);
+    if constexpr (WF_MIXED) {
+        __syncthreads();
+        wf_block_max_store(pabs, &sp.pmax[sp.npmax + blockIdx.x]);
+    }
 }
 
 /* ------------------------------------------------------------- launchers ---- */
@@ -1090,8 +1119,8 @@ hipError_t wf_launch_thomas(const WfSpectral &sp, hipStream_t s) {
     return hipGetLastError();
 }
 
-hipError_t wf_launch_iter_check(const WfSpectral &sp, double tol, int m, int max_iter, hipStream_t s) {
-    hipLaunchKernelGGL(k_iter_check, dim3(1), dim3(WF_NT), 0, s, sp, tol, m, max_iter);
+hipError_t wf_launch_iter_check(const WfSpectral &sp, double tol, double rtol, int m, int max_iter, hipStream_t s) {
+    hipLaunchKernelGGL(k_iter_check, dim3(1), dim3(WF_NT), 0, s, sp, tol, rtol, m, max_iter);
     return hipGetLastError();
 }
 

@@ -30,6 +30,10 @@
 #include "wf_kernels.h"
 
 #define WF_MAX_EVENTS 512
+/* mixed form: relative stopping tolerance of the pressure iteration */
+#ifndef WF_MIXED_PRESSURE_RTOL
+#define WF_MIXED_PRESSURE_RTOL 3e-5
+#endif
 
 #define WF_HIP(call)                                                                              \
     do {                                                                                          \
@@ -70,6 +74,7 @@ struct Solver {
     int *hflag_dev;  /* its device address */
     int method; /* 0 Euler, 1 RK2, 2 RK4 */
     long press_solves, steps;
+    double press_rtol; /* mixed form: relative stopping tolerance of the pressure iteration */
     double last_err;
     int last_iters;
     /* profiling */
@@ -459,8 +464,16 @@ static int wf_create_impl(Solver *s, const Parameters *P) {
     const int nchunks = (sp.nz + sp.thomas_lc - 1) / sp.thomas_lc;
     WF_HIP(hipMalloc((void **)&s->UB, sizeof(double) * (long)nchunks * sp.hx * sp.ldj));
     WF_HIP(hipMemset(s->UB, 0, sizeof(double) * (long)nchunks * sp.hx * sp.ldj));
-    WF_HIP(hipMalloc((void **)&s->pmax, sizeof(double) * sp.npmax));
-    WF_HIP(hipMemset(s->pmax, 0, sizeof(double) * sp.npmax));
+    WF_HIP(hipMalloc((void **)&s->pmax, sizeof(double) * 2 * sp.npmax));
+    WF_HIP(hipMemset(s->pmax, 0, sizeof(double) * 2 * sp.npmax));
+#if WF_MIXED
+    /* the FP32 iteration also stops once max|dp| <= rtol max|p|
+     * (WF_MIXED_PRESSURE_RTOL overrides the default) */
+    const char *ert = getenv("WF_MIXED_PRESSURE_RTOL");
+    s->press_rtol = ert ? atof(ert) : WF_MIXED_PRESSURE_RTOL;
+#else
+    s->press_rtol = 0.0;
+#endif
     sp.F = s->F;
     sp.Ptop = s->Ptop;
     sp.A = s->A;
@@ -668,7 +681,8 @@ static int wf_pressure_solve(Solver *s, char *Ys_buf, char *Yn_buf) {
         WF_LAUNCH(s, WF_K_THOMAS, wf_launch_thomas(s->sp, s->stream));
         WF_LAUNCH(s, WF_K_PFY, wf_launch_fy(s->sp, s->F, sstride, 0, s->sp.nz, +1, 1, s->stream));
         WF_LAUNCH(s, WF_K_PINV, wf_launch_inverse_fx(s->c, s->sp, s->p, s->stream));
-        WF_LAUNCH(s, WF_K_OTHER, wf_launch_iter_check(s->sp, P.pressure_solver_tol, m, max_iter, s->stream));
+        WF_LAUNCH(s, WF_K_OTHER,
+                  wf_launch_iter_check(s->sp, P.pressure_solver_tol, s->press_rtol, m, max_iter, s->stream));
         if (m + 1 >= max_iter) break;
         /* wait until the device is within WF_LOOKAHEAD solves; if the stream
          * drains without the flag advancing, fall through and keep queueing */
