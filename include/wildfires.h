@@ -351,6 +351,25 @@ int wf_solver_stats(const wf_solver *solver, long *pressure_iterations, long *st
  * reference logs them (src/cu/pressure.cu:597-602). */
 int wf_solver_last_pressure(const wf_solver *solver, double *error, int *iterations);
 
+/* Checkpoint reports of the current device state in one reduction on the
+ * device (src/cu/functions.cu:82-185): report[WF_REPORT_CFL..Y_MAX] are the
+ * values timestep_reports computes from the host state (bit for bit: maxima
+ * and minima), with its initial values (Y_min starts at 0).  *nan_index is -1,
+ * or the position ((i*Ny + j)*Nz + k)*5 + f of the first NaN in the reference's
+ * loop order (field f of u, v, w, T, Y), at which timestep_reports would stop.
+ * Two diagnostics have no reference counterpart: the resolved kinetic energy
+ * (mean of |u|^2/2 over the unique nodes of levels 1..Nz-2) and the fraction
+ * of fuel columns (unique, Nz_Y > 0) whose mean Y fell below 1/2. */
+#define WF_REPORT_CFL 0
+#define WF_REPORT_T_MIN 1
+#define WF_REPORT_T_MAX 2
+#define WF_REPORT_Y_MIN 3
+#define WF_REPORT_Y_MAX 4
+#define WF_REPORT_KINETIC_ENERGY 5
+#define WF_REPORT_BURNED 6
+#define WF_REPORT_COUNT 7
+int wf_solver_reports(wf_solver *solver, double *report, long *nan_index);
+
 /* Kernel classes timed by the profiler. */
 #define WF_K_DERIVED 0  /* per-stage derived-field pass */
 #define WF_K_STAGE 1    /* fused RHS + SGS + boundary + RK-combination pass */

@@ -1,0 +1,100 @@
+"""Checkpoint reports reduced on the device (wf_solver_reports) against the
+host restatement timestep_reports (src/cu/functions.cu:82-185) on the same
+state, and solve_PDE's log lines, which now come from the device reports.
+
+CFL and the fuel-box T and Y ranges are maxima and minima, so they must agree
+bit for bit; the kinetic energy (a sum) to 1e-12 relative."""
+import ctypes
+import glob
+import os
+import re
+
+import numpy as np
+import pytest
+
+from conftest import make_params
+
+pytestmark = pytest.mark.gpu
+
+
+def _host_reports(P, y):
+    from wildfires_amd import _abi
+    vals = [ctypes.c_double() for _ in range(5)]
+    y = np.ascontiguousarray(y, dtype=np.float64)
+    _abi.load().timestep_reports(y.ctypes.data_as(_abi.DBL_P), *[ctypes.byref(v) for v in vals], 1, P.p)
+    CFL, Y_min, Y_max, T_min, T_max = [v.value for v in vals]
+    return {"CFL": CFL, "T_min": T_min, "T_max": T_max, "Y_min": Y_min, "Y_max": Y_max}
+
+
+def _numpy_diagnostics(P, y):
+    u, v, w, T, Y = P.split(y)
+    ke = float(np.mean(0.5 * (u[:-1, :-1, 1:-1] ** 2 + v[:-1, :-1, 1:-1] ** 2 + w[:-1, :-1, 1:-1] ** 2)))
+    nzy = np.asarray(P.Nz_Y, dtype=np.int64).reshape(P.p.Nx, P.p.Ny)[:-1, :-1]
+    s = np.zeros_like(nzy, dtype=np.float64)
+    for k in range(Y.shape[2]):
+        s += np.where(k < nzy, Y[:-1, :-1, k], 0.0)
+    fuel = nzy > 0
+    burned = float(np.mean((s < 0.5 * nzy)[fuel])) if fuel.any() else 0.0
+    return ke, burned
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_device_reports_match_host(tmp_path, precision):
+    from wildfires_amd import api
+    P = make_params(tmp_path, "small_hill")
+    print("small_hill, 10 steps,", precision, flush=True)
+    y0, p0 = api.initial_conditions(P)
+    with api.Solver(P, precision) as S:
+        S.set_state(y0, p0)
+        S.step(1, 10)
+        rep = S.reports()
+        y, _ = S.get_state()
+    host = _host_reports(P, y)
+    print(rep, host)
+    for k, v in host.items():
+        assert rep[k] == v, (k, rep[k], v)
+    ke, burned = _numpy_diagnostics(P, y)
+    assert rep["kinetic_energy"] == pytest.approx(ke, rel=1e-12)
+    assert rep["burned"] == burned
+    assert rep["nan_index"] == -1
+
+
+def test_device_reports_find_the_first_nan(tmp_path):
+    from wildfires_amd import api
+    P = make_params(tmp_path, "small_flat")
+    y0, p0 = api.initial_conditions(P)
+    u, v, w, T, Y = P.split(y0)
+    w[5, 2, 3] = np.nan
+    T[4, 7, 1] = np.nan  # earlier in the reference's (i, j, k, field) order
+    print("small_flat with NaNs at w(5,2,3) and T(4,7,1)", flush=True)
+    with api.Solver(P) as S:
+        S.set_state(y0, p0)
+        rep = S.reports()
+    Ny, Nz = P.p.Ny, P.p.Nz
+    assert rep["nan_index"] == ((4 * Ny + 7) * Nz + 1) * 5 + 3
+
+
+def test_solve_pde_logs_the_device_reports(tmp_path):
+    """solve_PDE (the drop-in time loop) logs each checkpoint from the device
+    reports; the values equal timestep_reports of the snapshot it saves."""
+    from wildfires_amd import _abi, api
+    P = make_params(tmp_path, "small_flat", Nt=20, NT=10, t_max=0.2)
+    y0, p0 = api.initial_conditions(P)
+    print("solve_PDE small_flat, 20 steps, checkpoints at 10 and 20", flush=True)
+    y = np.ascontiguousarray(y0)
+    p = np.ascontiguousarray(p0)
+    _abi.load().solve_PDE(y.ctypes.data_as(_abi.DBL_P), p.ctypes.data_as(_abi.DBL_P), P.p)
+    save = P.save_path
+    log = open(os.path.join(save, "log.txt")).read()
+    blocks = re.findall(r"Time step: +(\d+).*?\nCFL: (\S+)\nTemperature: Min = (\S+), Max = (\S+)\n"
+                        r"Fuel: Min = (\S+), Max = (\S+)", log, re.S)
+    assert [int(b[0]) for b in blocks] == [10, 20]
+    n3 = P.p.Nx * P.p.Ny * P.p.Nz
+    for (n, cfl, tmin, tmax, ymin, ymax), idx in zip(blocks, (1, 2)):
+        snap = np.concatenate([np.fromfile(os.path.join(save, "%s.bin.%d" % (f, idx))) for f in "uvwTY"])
+        assert snap.size == P.S and np.fromfile(os.path.join(save, "u.bin.%d" % idx)).size == n3
+        h = _host_reports(P, snap)
+        assert cfl == "%f" % h["CFL"]
+        assert (tmin, tmax) == ("%.2f" % h["T_min"], "%.2f" % h["T_max"])
+        assert (ymin, ymax) == ("%.2f" % h["Y_min"], "%.2f" % h["Y_max"])
+    assert glob.glob(os.path.join(save, "p.bin.2"))

@@ -18,7 +18,7 @@ EXE = os.path.join(PKG, "wildfires_amd")
 HOST_SOURCES = ["params.c", "terrain.c", "init.c", "output.c", "solver.c"]
 # compiled once per precision: WF_MIXED=0 (namespace wf64, FP64) and WF_MIXED=1
 # (namespace wf32, the mixed form; any implicit FP32 -> FP64 promotion is an error)
-HIP_PRECISION_SOURCES = ["wf_stage.hip", "wf_pressure.hip", "wf_project.hip", "wf_solver.hip"]
+HIP_PRECISION_SOURCES = ["wf_stage.hip", "wf_pressure.hip", "wf_project.hip", "wf_reports.hip", "wf_solver.hip"]
 # compiled once: the C-ABI over both precisions and the closure trainer
 HIP_SOURCES = ["wf_api.hip", "wf_closure_train.hip"]
 MIXED_FLAGS = ["-DWF_MIXED=1", "-Wdouble-promotion", "-Werror=double-promotion"]

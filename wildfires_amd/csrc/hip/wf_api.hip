@@ -116,6 +116,10 @@ extern "C" int wf_solver_last_pressure(const wf_solver *h, double *error, int *i
 }
 extern "C" int wf_solver_set_closure(wf_solver *h, const float *params, long n) { WF_DISPATCH(h, set_closure, params, n); }
 
+extern "C" int wf_solver_reports(wf_solver *h, double *report, long *nan_index) {
+    WF_DISPATCH(h, reports, report, nan_index);
+}
+
 extern "C" int wf_device_synchronize(void) {
     WF_HIP(hipDeviceSynchronize());
     return WF_OK;

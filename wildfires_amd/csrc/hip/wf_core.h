@@ -34,6 +34,7 @@ int wf_closure_upload(const float *params, long n, float **prm_dev, void **frag_
     int kernel_times(const Solver *s, int kind, double *total_ms, long *launches);                            \
     int last_pressure(const Solver *s, double *error, int *iterations);                                       \
     int set_closure(Solver *s, const float *params, long n);                                                   \
+    int reports(Solver *s, double *out, long *nan_index);                                                      \
     }
 
 WF_CORE_DECLARE(wf64)

@@ -164,6 +164,14 @@ struct WfSource {
     const wf_real *T_src; /* mode 1: T at the start of the run (internal layout) */
 };
 
+/* checkpoint reports (wf_reports.hip): WF_REP_N partial values per workgroup,
+ * max |u|, |v|, |w|, min T, max T, min Y, max Y (fuel box), sum of the kinetic
+ * energy, burned fuel columns, fuel columns */
+#define WF_REP_N 10
+#define WF_REP_BLOCKS 2048
+hipError_t wf_launch_reports(const WfConst &c, const WfState &X, const int *in_box, double *part, int nblocks,
+                             unsigned long long *nan_first, hipStream_t s);
+
 /* projection pass (wf_project.hip) */
 hipError_t wf_launch_project(const WfConst &c, const WfState &Ys, const WfState &Yn, const wf_real *p,
                              const WfState &out, const WfSource &src, hipStream_t s);

@@ -75,6 +75,10 @@ struct Solver {
     int method; /* 0 Euler, 1 RK2, 2 RK4 */
     long press_solves, steps;
     double press_rtol; /* mixed form: relative stopping tolerance of the pressure iteration */
+    /* checkpoint reports: fuel-box column mask, per-workgroup partials, first NaN */
+    int *in_box;
+    double *rep_part;
+    unsigned long long *rep_nan;
     double last_err;
     int last_iters;
     /* profiling */
@@ -323,6 +327,15 @@ static int wf_create_impl(Solver *s, const Parameters *P) {
     if ((rc = wf_upload_real(&s->topo, topo.data(), s->plane))) return rc;
     if ((rc = wf_upload(&s->cut, cut.data(), s->plane))) return rc;
     if ((rc = wf_upload(&s->nzy, nzy.data(), s->plane))) return rc;
+    /* the fuel box of the checkpoint reports (src/cu/functions.cu:94-96) */
+    std::vector<int> in_box(s->plane);
+    for (int i = 0; i < Nx; i++)
+        for (int j = 0; j < Ny; j++)
+            in_box[i + (long)Nx * j] = P->Y0_x_start <= P->x[i] && P->x[i] <= P->Y0_x_end &&
+                                       P->Y0_y_start <= P->y[j] && P->y[j] <= P->Y0_y_end;
+    if ((rc = wf_upload(&s->in_box, in_box.data(), s->plane))) return rc;
+    WF_HIP(hipMalloc((void **)&s->rep_part, sizeof(double) * WF_REP_N * WF_REP_BLOCKS));
+    WF_HIP(hipMalloc((void **)&s->rep_nan, sizeof(unsigned long long)));
 
     WfConst &c = s->c;
     memset(&c, 0, sizeof(c));
@@ -534,7 +547,8 @@ int destroy(Solver *s) {
     void *bufs[] = {s->zc, s->derived, s->p, s->A, s->UB, s->pmax, s->F, s->Ptop, s->z, s->cd2, s->delta_k,
                     s->cl_prm, s->cl_frag, s->topo, s->kx2, s->ky2, s->ta,
                     s->tc, s->beta, s->x, s->y, s->cut, s->nzy, s->radx, s->rady, s->twx, s->twy, s->scratch,
-                    s->Tsrc, s->rho, s->ctl, s->kx2d, s->ky2d, s->tad, s->tcd, s->betad};
+                    s->Tsrc, s->rho, s->ctl, s->kx2d, s->ky2d, s->tad, s->tcd, s->betad,
+                    s->in_box, s->rep_part, s->rep_nan};
     for (size_t q = 0; q < sizeof(bufs) / sizeof(bufs[0]); q++)
         if (bufs[q]) (void)hipFree(bufs[q]);
     for (int q = 0; q < 2 * WF_MAX_EVENTS; q++)
@@ -826,6 +840,47 @@ int kernel_times(const Solver *s, int kind, double *total_ms, long *launches) {
     if (s == NULL || kind < 0 || kind >= WF_K_COUNT) return wf_fail(WF_E_ARG, "bad argument");
     if (total_ms) *total_ms = s->k_ms[kind];
     if (launches) *launches = s->k_count[kind];
+    return WF_OK;
+}
+
+/* Checkpoint reports of the current state (src/cu/functions.cu:82-185) from
+ * one device reduction: the host reduces the per-workgroup partials. */
+int reports(Solver *s, double *out, long *nan_index) {
+    if (s == NULL || out == NULL || nan_index == NULL) return wf_fail(WF_E_ARG, "null argument");
+    const Parameters &P = s->P;
+    const WfState x = wf_view(s, s->state[s->i_yn]);
+    const int nb = (int)((s->N + 255) / 256 < WF_REP_BLOCKS ? (s->N + 255) / 256 : WF_REP_BLOCKS);
+    WF_HIP(hipMemsetAsync(s->rep_nan, 0xff, sizeof(unsigned long long), s->stream));
+    WF_HIP(wf_launch_reports(s->c, x, s->in_box, s->rep_part, nb, s->rep_nan, s->stream));
+    std::vector<double> part((size_t)nb * WF_REP_N);
+    unsigned long long first = 0;
+    WF_HIP(hipMemcpyAsync(part.data(), s->rep_part, sizeof(double) * part.size(), hipMemcpyDeviceToHost, s->stream));
+    WF_HIP(hipMemcpyAsync(&first, s->rep_nan, sizeof(first), hipMemcpyDeviceToHost, s->stream));
+    WF_HIP(hipStreamSynchronize(s->stream));
+    /* the reference's initial values (src/cu/functions.cu:86-88) */
+    double mu = 0.0, mv = 0.0, mw = 0.0, t_lo = 1e9, t_hi = -1e9, y_lo = 0.0, y_hi = -1e9;
+    double ke = 0.0, burned = 0.0, cols = 0.0;
+    for (int b = 0; b < nb; b++) {
+        const double *r = &part[(size_t)b * WF_REP_N];
+        mu = (r[0] > mu) ? r[0] : mu;
+        mv = (r[1] > mv) ? r[1] : mv;
+        mw = (r[2] > mw) ? r[2] : mw;
+        t_lo = (r[3] < t_lo) ? r[3] : t_lo;
+        t_hi = (r[4] > t_hi) ? r[4] : t_hi;
+        y_lo = (r[5] < y_lo) ? r[5] : y_lo;
+        y_hi = (r[6] > y_hi) ? r[6] : y_hi;
+        ke += r[7];
+        burned += r[8];
+        cols += r[9];
+    }
+    out[WF_REPORT_CFL] = P.dt * (mu / P.dx + mv / P.dy + mw / P.dz);
+    out[WF_REPORT_T_MIN] = t_lo;
+    out[WF_REPORT_T_MAX] = t_hi;
+    out[WF_REPORT_Y_MIN] = y_lo;
+    out[WF_REPORT_Y_MAX] = y_hi;
+    out[WF_REPORT_KINETIC_ENERGY] = ke / ((double)(P.Nx - 1) * (P.Ny - 1) * (P.Nz - 2));
+    out[WF_REPORT_BURNED] = (cols > 0.0) ? burned / cols : 0.0;
+    *nan_index = (first == ~0ull) ? -1 : (long)first;
     return WF_OK;
 }
 

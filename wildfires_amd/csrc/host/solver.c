@@ -21,6 +21,23 @@ static void wf_fatal(Parameters parameters, const char *what) {
     exit(1);
 }
 
+/* The NaN exit of timestep_reports (src/cu/functions.cu:141-165) for the first
+ * NaN the device reduction found, at position ((i Ny + j) Nz + k) 5 + f. */
+static void wf_nan_exit(Parameters P, long at, int n) {
+    static const char *names[5] = {"u", "v", "w", "T", "Y"};
+    char message[256];
+    const int f = (int)(at % 5);
+    at /= 5;
+    const int k = (int)(at % P.Nz);
+    at /= P.Nz;
+    const int j = (int)(at % P.Ny);
+    const int i = (int)(at / P.Ny);
+    sprintf(message, "NaN value found in %s(%f, %f, %f, %f) in node (%d, %d, %d) at timestep %d (%d). Exiting...",
+            names[f], P.x[i], P.y[j], P.z[k], P.t[n], i, j, k, n, n / P.NT);
+    log_message(P, message);
+    exit(1);
+}
+
 void solve_PDE(double *y_n, double *p, Parameters parameters) {
     int Nt = parameters.Nt, NT = parameters.NT;
     long size = wf_state_size(&parameters);
@@ -56,9 +73,18 @@ void solve_PDE(double *y_n, double *p, Parameters parameters) {
         if (parameters.time_step_log > 0 && n % parameters.time_step_log == 0)
             printf("Time step: %d, Average time: %lf s\n", n, cum_step_time / n);
         if (n % NT == 0 || n == Nt) {
-            if (wf_solver_get_state(solver, y_host, p_host) != WF_OK) wf_fatal(parameters, "Device state download failed");
-            timestep_reports(y_host, &CFL, &Y_min, &Y_max, &T_min, &T_max, n, parameters);
+            /* the reports of timestep_reports, reduced on the device */
+            double rep[WF_REPORT_COUNT];
+            long nan_at = -1;
+            if (wf_solver_reports(solver, rep, &nan_at) != WF_OK) wf_fatal(parameters, "Device reports failed");
+            if (nan_at >= 0) wf_nan_exit(parameters, nan_at, n);
+            CFL = rep[WF_REPORT_CFL];
+            T_min = rep[WF_REPORT_T_MIN];
+            T_max = rep[WF_REPORT_T_MAX];
+            Y_min = rep[WF_REPORT_Y_MIN];
+            Y_max = rep[WF_REPORT_Y_MAX];
             log_timestep(parameters, n, t[n], step_time, CFL, T_min, T_max, Y_min, Y_max);
+            if (wf_solver_get_state(solver, y_host, p_host) != WF_OK) wf_fatal(parameters, "Device state download failed");
             if (parameters.variable_density == 1) {
                 sprintf(message, "Pressure solver: Error = %e, iterations = %d", error, iterations);
                 log_message(parameters, message);
