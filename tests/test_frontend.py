@@ -246,3 +246,39 @@ def test_solver_precision_is_checked_before_the_device(tmp_path):
     assert b"precision" in lib.wf_last_error()
     with pytest.raises(ValueError):
         api.Solver(prm, precision="fp16")
+
+
+def test_output_reader_round_trip(tmp_path):
+    """save_domain and save_data (the C writers) read back through
+    wildfires_amd.output with the reference converter's conventions: arrays
+    (Nt, Ny, Nx, Nz) over the checkpoints of t.bin, Y zero above Nz_Y_max."""
+    from wildfires_amd import output
+    P = make_params(tmp_path, "small_hill", Nt=20, NT=10, t_max=0.2)
+    lib = _abi.load()
+    lib.save_domain(P.p)
+    rng = np.random.default_rng(5)
+    states = []
+    for n in range(3):
+        y = rng.standard_normal(P.S)
+        p = rng.standard_normal(P.N)
+        lib.save_data(y.ctypes.data_as(_abi.DBL_P), p.ctypes.data_as(_abi.DBL_P), n, P.p)
+        states.append((y, p))
+    fields, domain = output.load_output(P.save_path)
+    assert domain["t"].size == 3 and np.array_equal(domain["x"], P.x)
+    Nx, Ny, Nz, Nzm = P.p.Nx, P.p.Ny, P.p.Nz, P.p.Nz_Y_max
+    for n, (y, p) in enumerate(states):
+        u, v, w, T, Y = P.split(y)
+        for name, a in (("u", u), ("v", v), ("w", w), ("T", T)):
+            assert fields[name].shape == (3, Ny, Nx, Nz)
+            assert np.array_equal(fields[name][n], a.transpose(1, 0, 2)), name
+        assert np.array_equal(fields["Y"][n][:, :, :Nzm], Y.transpose(1, 0, 2))
+        assert not fields["Y"][n][:, :, Nzm:].any()
+        assert np.array_equal(fields["p"][n], p.reshape(Nx, Ny, Nz).transpose(1, 0, 2))
+    output.convert(P.save_path, str(tmp_path / "npz"))
+    with np.load(str(tmp_path / "npz" / "data.npz")) as d:
+        assert sorted(d.files) == sorted(output.FIELDS)
+    # a truncated snapshot is an error, not a silent reshape
+    fn = os.path.join(P.save_path, "T.bin.1")
+    open(fn, "r+b").truncate(8 * (P.N - 1))
+    with pytest.raises(ValueError):
+        output.load_output(P.save_path)
