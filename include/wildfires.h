@@ -370,6 +370,16 @@ int wf_solver_last_pressure(const wf_solver *solver, double *error, int *iterati
 #define WF_REPORT_COUNT 7
 int wf_solver_reports(wf_solver *solver, double *report, long *nan_index);
 
+/* Asynchronous checkpoint of the current state (no reference counterpart;
+ * what solve_PDE uses so that a checkpoint's download and file writing overlap
+ * the following steps).  begin queues the state's conversion to the reference
+ * layout behind the work already queued and its download into a pinned host
+ * buffer the solver owns; stepping may continue at once.  wait blocks until
+ * that download has landed and returns host pointers to the state (length S)
+ * and p (Nx*Ny*Nz), valid until the next begin. */
+int wf_solver_checkpoint_begin(wf_solver *solver);
+int wf_solver_checkpoint_wait(wf_solver *solver, const double **y, const double **p);
+
 /* Kernel classes timed by the profiler. */
 #define WF_K_DERIVED 0  /* per-stage derived-field pass */
 #define WF_K_STAGE 1    /* fused RHS + SGS + boundary + RK-combination pass */

@@ -3,7 +3,9 @@ host restatement timestep_reports (src/cu/functions.cu:82-185) on the same
 state, and solve_PDE's log lines, which now come from the device reports.
 
 CFL and the fuel-box T and Y ranges are maxima and minima, so they must agree
-bit for bit; the kinetic energy (a sum) to 1e-12 relative."""
+bit for bit; the kinetic energy (a sum) to 1e-12 relative.  The asynchronous
+checkpoints (download and files overlapping later steps) must hold the state
+of their own step, bit for bit."""
 import ctypes
 import glob
 import os
@@ -81,8 +83,8 @@ def test_solve_pde_logs_the_device_reports(tmp_path):
     P = make_params(tmp_path, "small_flat", Nt=20, NT=10, t_max=0.2)
     y0, p0 = api.initial_conditions(P)
     print("solve_PDE small_flat, 20 steps, checkpoints at 10 and 20", flush=True)
-    y = np.ascontiguousarray(y0)
-    p = np.ascontiguousarray(p0)
+    y = y0.copy()  # solve_PDE writes the final state into its arguments
+    p = p0.copy()
     _abi.load().solve_PDE(y.ctypes.data_as(_abi.DBL_P), p.ctypes.data_as(_abi.DBL_P), P.p)
     save = P.save_path
     log = open(os.path.join(save, "log.txt")).read()
@@ -98,3 +100,38 @@ def test_solve_pde_logs_the_device_reports(tmp_path):
         assert (tmin, tmax) == ("%.2f" % h["T_min"], "%.2f" % h["T_max"])
         assert (ymin, ymax) == ("%.2f" % h["Y_min"], "%.2f" % h["Y_max"])
     assert glob.glob(os.path.join(save, "p.bin.2"))
+    # the snapshots are the states of steps 10 and 20 (written while later
+    # steps ran): step 20 is solve_PDE's final state, step 10 a rerun's
+    snap2 = np.concatenate([np.fromfile(os.path.join(save, "%s.bin.2" % f)) for f in "uvwTY"])
+    assert np.array_equal(snap2, y)
+    with api.Solver(P) as S:
+        S.set_state(y0, p0)
+        S.step(1, 10)
+        y10, p10 = S.get_state()
+    snap1 = np.concatenate([np.fromfile(os.path.join(save, "%s.bin.1" % f)) for f in "uvwTY"])
+    assert np.array_equal(snap1, y10)
+    assert np.array_equal(np.fromfile(os.path.join(save, "p.bin.1")), p10)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_checkpoint_survives_the_steps_after_it(tmp_path, precision):
+    """wf_solver_checkpoint_begin, then more steps, then wait: the host copy is
+    the state at the checkpoint."""
+    from wildfires_amd import _abi, api
+    P = make_params(tmp_path, "small_hill")
+    y0, p0 = api.initial_conditions(P)
+    lib = _abi.load()
+    print("checkpoint after 5 steps, 5 more queued,", precision, flush=True)
+    with api.Solver(P, precision) as S:
+        S.set_state(y0, p0)
+        S.step(1, 5)
+        y5, p5 = S.get_state()
+        assert lib.wf_solver_checkpoint_begin(S.handle) == 0
+        S.step(6, 5)
+        yp, pp = _abi.DBL_P(), _abi.DBL_P()
+        assert lib.wf_solver_checkpoint_wait(S.handle, ctypes.byref(yp), ctypes.byref(pp)) == 0
+        yc = np.ctypeslib.as_array(yp, shape=(P.S,)).copy()
+        pc = np.ctypeslib.as_array(pp, shape=(P.N,)).copy()
+        y10, _ = S.get_state()
+    assert np.array_equal(yc, y5) and np.array_equal(pc, p5)
+    assert not np.array_equal(y10, y5)

@@ -28,7 +28,7 @@ MIXED_FLAGS = ["-DWF_MIXED=1", "-Wdouble-promotion", "-Werror=double-promotion"]
 HIPCC_FLAGS = ["--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17", "-ffp-contract=off",
                "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
                "-Wno-unused-result", "-I" + os.path.join(ROOT, "include")]
-GCC_FLAGS = ["-O2", "-fPIC", "-std=gnu11", "-ffp-contract=off", "-fno-fast-math", "-Wall",
+GCC_FLAGS = ["-O2", "-fPIC", "-pthread", "-std=gnu11", "-ffp-contract=off", "-fno-fast-math", "-Wall",
              "-Wno-unused-result", "-Wno-format-truncation", "-I" + os.path.join(ROOT, "include")]
 
 
@@ -61,7 +61,8 @@ def build(verbose=False, jobs=None):
             objs.append(obj)
     with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
         list(ex.map(lambda c: _run(c, verbose), cmds))
-    _run([_hipcc(), "--offload-arch=gfx950", "-shared", "-fPIC", "-o", LIB + ".tmp"] + objs + ["-lm"], verbose)
+    _run([_hipcc(), "--offload-arch=gfx950", "-shared", "-fPIC", "-o", LIB + ".tmp"] + objs + ["-lm", "-lpthread"],
+         verbose)
     os.replace(LIB + ".tmp", LIB)
     main_obj = os.path.join(OUT, "main.c.o")
     _run(["gcc"] + GCC_FLAGS + ["-c", os.path.join(CSRC, "host", "main.c"), "-o", main_obj], verbose)

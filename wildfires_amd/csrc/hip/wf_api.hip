@@ -120,6 +120,14 @@ extern "C" int wf_solver_reports(wf_solver *h, double *report, long *nan_index) 
     WF_DISPATCH(h, reports, report, nan_index);
 }
 
+extern "C" int wf_solver_checkpoint_begin(wf_solver *h) {
+    if (h == NULL) return wf_fail(WF_E_ARG, "null solver");
+    return (h->precision == WF_PRECISION_MIXED) ? wf32::checkpoint_begin(h->s32) : wf64::checkpoint_begin(h->s64);
+}
+extern "C" int wf_solver_checkpoint_wait(wf_solver *h, const double **y, const double **p) {
+    WF_DISPATCH(h, checkpoint_wait, y, p);
+}
+
 extern "C" int wf_device_synchronize(void) {
     WF_HIP(hipDeviceSynchronize());
     return WF_OK;

@@ -79,6 +79,13 @@ struct Solver {
     int *in_box;
     double *rep_part;
     unsigned long long *rep_nan;
+    /* asynchronous checkpoints (allocated at the first one): the state and p in
+     * the reference layout on the device and in pinned host memory, the copy
+     * stream and its events */
+    double *ck_dev, *ck_host;
+    hipStream_t ck_stream;
+    hipEvent_t ck_conv, ck_done;
+    int ck_pending;
     double last_err;
     int last_iters;
     /* profiling */
@@ -553,6 +560,12 @@ int destroy(Solver *s) {
         if (bufs[q]) (void)hipFree(bufs[q]);
     for (int q = 0; q < 2 * WF_MAX_EVENTS; q++)
         if (s->ev[q]) (void)hipEventDestroy(s->ev[q]);
+    if (s->ck_stream) (void)hipStreamSynchronize(s->ck_stream);
+    if (s->ck_dev) (void)hipFree(s->ck_dev);
+    if (s->ck_host) (void)hipHostFree(s->ck_host);
+    if (s->ck_conv) (void)hipEventDestroy(s->ck_conv);
+    if (s->ck_done) (void)hipEventDestroy(s->ck_done);
+    if (s->ck_stream) (void)hipStreamDestroy(s->ck_stream);
     if (s->hflag) (void)hipHostFree(s->hflag);
     if (s->stream) (void)hipStreamDestroy(s->stream);
     free(s);
@@ -840,6 +853,46 @@ int kernel_times(const Solver *s, int kind, double *total_ms, long *launches) {
     if (s == NULL || kind < 0 || kind >= WF_K_COUNT) return wf_fail(WF_E_ARG, "bad argument");
     if (total_ms) *total_ms = s->k_ms[kind];
     if (launches) *launches = s->k_count[kind];
+    return WF_OK;
+}
+
+/* Asynchronous checkpoint of the current state: its conversion to the
+ * reference layout is queued on the compute stream (so later steps cannot
+ * overwrite it first), its download on a copy stream into pinned host memory.
+ * The caller continues stepping; checkpoint_wait returns the host copy once it
+ * has landed.  A checkpoint waits for the previous one's download. */
+int checkpoint_begin(Solver *s) {
+    if (s == NULL) return wf_fail(WF_E_ARG, "null solver");
+    const Parameters &P = s->P;
+    const size_t count = (size_t)(s->S + s->N);
+    if (s->ck_dev == NULL) {
+        WF_HIP(hipMalloc((void **)&s->ck_dev, sizeof(double) * count));
+        WF_HIP(hipHostMalloc((void **)&s->ck_host, sizeof(double) * count, hipHostMallocDefault));
+        WF_HIP(hipStreamCreate(&s->ck_stream));
+        WF_HIP(hipEventCreateWithFlags(&s->ck_conv, hipEventDisableTiming));
+        WF_HIP(hipEventCreateWithFlags(&s->ck_done, hipEventDisableTiming));
+    }
+    if (s->ck_pending) WF_HIP(hipStreamWaitEvent(s->stream, s->ck_done, 0));
+    const WfState v = wf_view(s, s->state[s->i_yn]);
+    const wf_real *f4[4] = {v.u, v.v, v.w, v.T};
+    for (int f = 0; f < 4; f++)
+        WF_HIP(wf_launch_to_reference(f4[f], s->ck_dev + f * s->N, P.Nx, P.Ny, P.Nz, s->stream));
+    WF_HIP(wf_launch_to_reference((const double *)v.Y, s->ck_dev + 4 * s->N, P.Nx, P.Ny, P.Nz_Y_max, s->stream));
+    WF_HIP(wf_launch_to_reference((const wf_real *)s->p, s->ck_dev + s->S, P.Nx, P.Ny, P.Nz, s->stream));
+    WF_HIP(hipEventRecord(s->ck_conv, s->stream));
+    WF_HIP(hipStreamWaitEvent(s->ck_stream, s->ck_conv, 0));
+    WF_HIP(hipMemcpyAsync(s->ck_host, s->ck_dev, sizeof(double) * count, hipMemcpyDeviceToHost, s->ck_stream));
+    WF_HIP(hipEventRecord(s->ck_done, s->ck_stream));
+    s->ck_pending = 1;
+    return WF_OK;
+}
+
+int checkpoint_wait(Solver *s, const double **y, const double **p) {
+    if (s == NULL || y == NULL || p == NULL) return wf_fail(WF_E_ARG, "null argument");
+    if (!s->ck_pending) return wf_fail(WF_E_ARG, "no checkpoint was started");
+    WF_HIP(hipEventSynchronize(s->ck_done));
+    *y = s->ck_host;
+    *p = s->ck_host + s->S;
     return WF_OK;
 }
 

@@ -1,8 +1,11 @@
 /*
  * solver.c -- solve_PDE: the time loop of src/cu/solver.cu:95-298 driven
  * through the wf_* entry points, with the reference's timing lines,
- * checkpoint reports, logs and snapshot files.
+ * checkpoint reports, logs and snapshot files.  Checkpoint reports come from
+ * one device reduction; a checkpoint's download and files overlap the steps
+ * that follow it.
  */
+#include <pthread.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -38,10 +41,35 @@ static void wf_nan_exit(Parameters P, long at, int n) {
     exit(1);
 }
 
+/* Checkpoint files written by a thread of their own while the next steps run:
+ * it waits for the checkpoint's download (wf_solver_checkpoint_wait) and
+ * writes the snapshot (save_data). */
+typedef struct {
+    wf_solver *solver;
+    Parameters parameters;
+    int index;
+    int rc;
+    pthread_t thread;
+    int active;
+} wf_writer;
+
+static void *wf_writer_main(void *arg) {
+    wf_writer *w = (wf_writer *)arg;
+    const double *y = NULL, *p = NULL;
+    w->rc = wf_solver_checkpoint_wait(w->solver, &y, &p);
+    if (w->rc == WF_OK) save_data((double *)y, (double *)p, w->index, w->parameters);
+    return NULL;
+}
+
+static void wf_writer_join(wf_writer *w) {
+    if (!w->active) return;
+    pthread_join(w->thread, NULL);
+    w->active = 0;
+    if (w->rc != WF_OK) wf_fatal(w->parameters, "Device state download failed");
+}
+
 void solve_PDE(double *y_n, double *p, Parameters parameters) {
     int Nt = parameters.Nt, NT = parameters.NT;
-    long size = wf_state_size(&parameters);
-    long n3 = (long)parameters.Nx * parameters.Ny * parameters.Nz;
     double *t = parameters.t;
     double CFL = 0.0, T_min = 1e9, T_max = -1e9, Y_min = 1e9, Y_max = -1e9;
     double cum_step_time = 0.0, step_time = 0.0, error = 0.0;
@@ -51,8 +79,10 @@ void solve_PDE(double *y_n, double *p, Parameters parameters) {
     wf_solver *solver = NULL;
     if (wf_solver_create(&parameters, &solver) != WF_OK) wf_fatal(parameters, "Device setup failed");
     if (wf_solver_set_state(solver, y_n, p) != WF_OK) wf_fatal(parameters, "Device state upload failed");
-    double *y_host = (double *)malloc(sizeof(double) * (size_t)size);
-    double *p_host = (double *)malloc(sizeof(double) * (size_t)n3);
+    wf_writer writer;
+    memset(&writer, 0, sizeof(writer));
+    writer.solver = solver;
+    writer.parameters = parameters;
     gettimeofday(&start_solver, NULL);
     for (int n = 1; n <= Nt; n++) {
         gettimeofday(&start_ts, NULL);
@@ -84,21 +114,29 @@ void solve_PDE(double *y_n, double *p, Parameters parameters) {
             Y_min = rep[WF_REPORT_Y_MIN];
             Y_max = rep[WF_REPORT_Y_MAX];
             log_timestep(parameters, n, t[n], step_time, CFL, T_min, T_max, Y_min, Y_max);
-            if (wf_solver_get_state(solver, y_host, p_host) != WF_OK) wf_fatal(parameters, "Device state download failed");
+            /* the previous snapshot is on disk before this one is queued */
+            wf_writer_join(&writer);
+            if (wf_solver_checkpoint_begin(solver) != WF_OK) wf_fatal(parameters, "Device state download failed");
             if (parameters.variable_density == 1) {
                 sprintf(message, "Pressure solver: Error = %e, iterations = %d", error, iterations);
                 log_message(parameters, message);
             }
-            save_data(y_host, p_host, n / NT, parameters);
+            writer.index = n / NT;
+            writer.rc = WF_OK;
+            if (pthread_create(&writer.thread, NULL, wf_writer_main, &writer) != 0) {
+                wf_writer_main(&writer); /* no thread: write it now */
+                if (writer.rc != WF_OK) wf_fatal(parameters, "Device state download failed");
+            } else {
+                writer.active = 1;
+            }
             printf("\n");
         }
     }
+    wf_writer_join(&writer);
     gettimeofday(&end_solver, NULL);
     format_seconds(wf_elapsed(&start_solver, &end_solver), formatted);
     sprintf(message, "\nSolver time: %s", formatted);
     log_message(parameters, message);
     if (wf_solver_get_state(solver, y_n, p) != WF_OK) wf_fatal(parameters, "Device state download failed");
-    free(y_host);
-    free(p_host);
     wf_solver_destroy(solver);
 }
