@@ -74,11 +74,29 @@ struct __attribute__((aligned(16))) WfClosureLds { /* fragments are read as 16-b
     float z[64];
 };
 
-/* with the hardware exponential and reciprocal (v_exp_f32, v_rcp_f32, about
- * 1 ulp each): 128 of these per node would otherwise dominate the network's
- * cost as full-precision division sequences */
+/* h / (1 + e^(-1.702 h)) with the hardware base-2 exponential and reciprocal
+ * (v_exp_f32, v_rcp_f32, about 1 ulp each; full-precision division would
+ * dominate the network's cost), the two constants folded into one factor:
+ * e^(-1.702 h) = 2^(h (-1.702 log2 e)).  v_exp_f32 takes the whole range
+ * (2^x = 0 or +inf at the ends), so h -> -inf gives -0 and h -> +inf gives h. */
+#define WF_CL_GELU_K (-1.702f * 1.44269504088896341f)
 __device__ __forceinline__ float wf_cl_gelu(float h) {
-    return h * __builtin_amdgcn_rcpf(1.0f + __expf(-1.702f * h));
+    return h * __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(h * WF_CL_GELU_K));
+}
+
+/* GELU of a bias-shifted accumulator quadruple d[0..3] + b, rounded to bf16
+ * two at a time (one packed conversion per pair) and stored at rows
+ * base, base + 64, base + 128, base + 192 of an LDS activation block. */
+typedef __bf16 wf_bf16x2 __attribute__((ext_vector_type(2)));
+typedef float wf_f32x2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ void wf_cl_gelu_store4(__bf16 *dst, const float d[4], float b) {
+#pragma unroll
+    for (int r = 0; r < 4; r += 2) {
+        const wf_f32x2 g = {wf_cl_gelu(d[r] + b), wf_cl_gelu(d[r + 1] + b)};
+        const wf_bf16x2 v = __builtin_convertvector(g, wf_bf16x2);
+        dst[r * 64] = v[0];
+        dst[(r + 1) * 64] = v[1];
+    }
 }
 
 /* Exchange through this wavefront's own LDS block: DS instructions of one
@@ -139,8 +157,8 @@ __device__ __forceinline__ float wf_cl_forward(const WfClosure &cl, WfClosureLds
         for (int nt = 0; nt < 4; nt++) {
             const int n = nt * 16 + lr;
             const float b = prm[WF_CL_B1 + n];
-#pragma unroll
-            for (int r = 0; r < 4; r++) L.h[(4 * lg + r) * 64 + n] = (__bf16)wf_cl_gelu(d[nt][r] + b);
+            const float q[4] = {d[nt][0], d[nt][1], d[nt][2], d[nt][3]};
+            wf_cl_gelu_store4(&L.h[(4 * lg) * 64 + n], q, b);
         }
         wf_cl_wave_sync();
         const wf_bf16x8 a2a = *(const wf_bf16x8 *)&L.h[lr * 64 + 8 * lg];
@@ -156,8 +174,8 @@ __device__ __forceinline__ float wf_cl_forward(const WfClosure &cl, WfClosureLds
         for (int nt = 0; nt < 4; nt++) {
             const int n = nt * 16 + lr;
             const float b = prm[WF_CL_B2 + n];
-#pragma unroll
-            for (int r = 0; r < 4; r++) L.h[(4 * lg + r) * 64 + n] = (__bf16)wf_cl_gelu(d[nt][r] + b);
+            const float q[4] = {d[nt][0], d[nt][1], d[nt][2], d[nt][3]};
+            wf_cl_gelu_store4(&L.h[(4 * lg) * 64 + n], q, b);
         }
         wf_cl_wave_sync();
         const wf_bf16x8 a3a = *(const wf_bf16x8 *)&L.h[lr * 64 + 8 * lg];
