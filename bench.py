@@ -199,6 +199,7 @@ def mixed_leg(params, y0, p0, N, args, y_ref, rate64, block):
     n += args.steps
     s1 = S.stats()
     y, p = S.get_state()
+    learned = S.reports()["learned"]
     S.profile(True)
     S.step(n, 1)
     times = S.kernel_times()
@@ -212,6 +213,8 @@ def mixed_leg(params, y0, p0, N, args, y_ref, rate64, block):
            "pressure_solves_per_step": round((s1["pressure_solves"] - s0["pressure_solves"]) / args.steps, 3),
            "kernels_mean_us": {k: round(1000.0 * ms / c, 2) for k, (ms, c) in times.items() if c},
            "state_finite": bool(np.isfinite(y).all())}
+    if block is not None:
+        out["learned_fraction"] = round(learned, 4)  # interior nodes on the network at the last stage
     if y_ref is not None:
         def rel(a, b):
             d = np.linalg.norm(b)
@@ -250,6 +253,7 @@ def closure_leg(solver, params, y0, p0, N, args):
     times = solver.kernel_times()
     solver.profile(False)
     y, _ = solver.get_state()
+    learned = solver.reports()["learned"]
     solver.set_closure(None)
     rate = N * args.steps / elapsed / 1e6
     ms, cnt = times.get("derived", (0.0, 0))
@@ -258,6 +262,7 @@ def closure_leg(solver, params, y0, p0, N, args):
             "dtype": "f64 state, bf16 x bf16 -> f32 MFMA closure network", "weights": weights,
             "pressure_solves_per_step": round((s1["pressure_solves"] - s0["pressure_solves"]) / args.steps, 3),
             "derived_mean_us": round(1000.0 * ms / cnt, 2) if cnt else None,
+            "learned_fraction": round(learned, 4),
             "state_finite": bool(np.isfinite(y).all())}
 
 

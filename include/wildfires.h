@@ -359,7 +359,9 @@ int wf_solver_last_pressure(const wf_solver *solver, double *error, int *iterati
  * loop order (field f of u, v, w, T, Y), at which timestep_reports would stop.
  * Two diagnostics have no reference counterpart: the resolved kinetic energy
  * (mean of |u|^2/2 over the unique nodes of levels 1..Nz-2) and the fraction
- * of fuel columns (unique, Nz_Y > 0) whose mean Y fell below 1/2. */
+ * of fuel columns (unique, Nz_Y > 0) whose mean Y fell below 1/2; and the
+ * fraction of those nodes whose last stage took the learned closure's
+ * viscosity (0 without a closure). */
 #define WF_REPORT_CFL 0
 #define WF_REPORT_T_MIN 1
 #define WF_REPORT_T_MAX 2
@@ -367,7 +369,8 @@ int wf_solver_last_pressure(const wf_solver *solver, double *error, int *iterati
 #define WF_REPORT_Y_MAX 4
 #define WF_REPORT_KINETIC_ENERGY 5
 #define WF_REPORT_BURNED 6
-#define WF_REPORT_COUNT 7
+#define WF_REPORT_LEARNED 7 /* fraction of interior unique nodes whose last stage used the learned closure */
+#define WF_REPORT_COUNT 8
 int wf_solver_reports(wf_solver *solver, double *report, long *nan_index);
 
 /* Asynchronous checkpoint of the current state (no reference counterpart;

@@ -182,8 +182,8 @@ def test_trained_closure_a_posteriori():
     block = closure.load(TRAINED)
     with open(os.path.join(os.path.dirname(TRAINED), "closure_report.json")) as f:
         rep = json.load(f)
-    r = len(rep["runs"]) - rep["held_out"]  # the first held-out run
-    u_r, x_ign = rep["runs"][r]["u_r"], rep["runs"][r]["x_ignition"]
+    run = next(r for r in rep["runs"] if r["held_out"] and r["scale"] == "64")  # a held-out run, 64 x 64 x 32
+    u_r, x_ign = run["u_r"], run["x_ignition"]
     print("held-out scenario u_r %.3f m/s, line at x %.2f m: fine run" % (u_r, x_ign), flush=True)
     from conftest import short_dir
     wd = short_dir()  # the output path must fit the 64-byte field of Parameters

@@ -207,12 +207,13 @@ class Solver:
     def reports(self):
         """Checkpoint reports of the current state (wf_solver_reports): CFL,
         T_min, T_max, Y_min, Y_max (as timestep_reports), kinetic_energy,
-        burned (fraction of fuel columns below Y = 1/2), nan_index (-1 or the
-        first NaN's position in the reference's loop order)."""
-        rep = np.zeros(7)
+        burned (fraction of fuel columns below Y = 1/2), learned (fraction of
+        interior nodes on the learned closure at the last stage), nan_index
+        (-1 or the first NaN's position in the reference's loop order)."""
+        rep = np.zeros(8)
         nan = ctypes.c_long(-1)
         _check(lib().wf_solver_reports(self.handle, _dp(rep), ctypes.byref(nan)))
-        keys = ("CFL", "T_min", "T_max", "Y_min", "Y_max", "kinetic_energy", "burned")
+        keys = ("CFL", "T_min", "T_max", "Y_min", "Y_max", "kinetic_energy", "burned", "learned")
         out = {k: float(v) for k, v in zip(keys, rep)}
         out["nan_index"] = int(nan.value)
         return out

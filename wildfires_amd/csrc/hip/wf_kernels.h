@@ -166,11 +166,11 @@ struct WfSource {
 
 /* checkpoint reports (wf_reports.hip): WF_REP_N partial values per workgroup,
  * max |u|, |v|, |w|, min T, max T, min Y, max Y (fuel box), sum of the kinetic
- * energy, burned fuel columns, fuel columns */
-#define WF_REP_N 10
+ * energy, burned fuel columns, fuel columns, nodes with a learned viscosity */
+#define WF_REP_N 11
 #define WF_REP_BLOCKS 2048
-hipError_t wf_launch_reports(const WfConst &c, const WfState &X, const int *in_box, double *part, int nblocks,
-                             unsigned long long *nan_first, hipStream_t s);
+hipError_t wf_launch_reports(const WfConst &c, const WfState &X, const wf_real *mu_sgs, const int *in_box,
+                             double *part, int nblocks, unsigned long long *nan_first, hipStream_t s);
 
 /* projection pass (wf_project.hip) */
 hipError_t wf_launch_project(const WfConst &c, const WfState &Ys, const WfState &Yn, const wf_real *p,

@@ -4,7 +4,9 @@
  * minimum and maximum of T and of Y over the nodes inside the initial fuel
  * box, and the first non-finite value in the reference's loop order; plus two
  * diagnostics of the flow itself, the resolved kinetic energy and the
- * fraction of fuel columns whose mean fuel fraction fell below one half.
+ * fraction of fuel columns whose mean fuel fraction fell below one half, and
+ * the fraction of interior nodes whose last derived pass used the learned
+ * closure.
  *
  * Every workgroup reduces its nodes (grid-stride) and writes WF_REP_N partial
  * values; the host reduces the partials (a few thousand) in FP64.  Minima and
@@ -40,12 +42,13 @@ __device__ __forceinline__ double wf_rep_reduce(double v, double *red) {
  * against the initial fuel box; part: WF_REP_N partials per workgroup;
  * nan_first: the smallest ((i Ny + j) Nz + k) 5 + f of a NaN (field f of
  * u, v, w, T, Y), initialised to ~0 by the caller. */
-__global__ __launch_bounds__(WF_REP_NT) void k_reports(WfConst c, WfState X, const int *in_box, double *part,
+__global__ __launch_bounds__(WF_REP_NT) void k_reports(WfConst c, WfState X, const wf_real *mu_sgs,
+                                                       const int *in_box, double *part,
                                                        unsigned long long *nan_first) {
     __shared__ double red[WF_REP_NT / 64];
     double mu = 0.0, mv = 0.0, mw = 0.0;
     double t_lo = 1e9, t_hi = -1e9, y_lo = 1e9, y_hi = -1e9;
-    double ke = 0.0, burned = 0.0, fuel_cols = 0.0;
+    double ke = 0.0, burned = 0.0, fuel_cols = 0.0, learned = 0.0;
     const long N = c.N;
     for (long q = (long)blockIdx.x * WF_REP_NT + threadIdx.x; q < N; q += (long)gridDim.x * WF_REP_NT) {
         const int k = (int)(q / c.plane);
@@ -71,7 +74,11 @@ __global__ __launch_bounds__(WF_REP_NT) void k_reports(WfConst c, WfState X, con
             }
         }
         /* resolved kinetic energy over the unique nodes of the interior levels */
-        if (i < c.Nx - 1 && j < c.Ny - 1 && k >= 1 && k <= c.Nz - 2) ke += 0.5 * (u * u + v * v + w * w);
+        if (i < c.Nx - 1 && j < c.Ny - 1 && k >= 1 && k <= c.Nz - 2) {
+            ke += 0.5 * (u * u + v * v + w * w);
+            /* the last derived pass stored a learned viscosity negative */
+            learned += (mu_sgs[q] < WR(0.0)) ? 1.0 : 0.0;
+        }
         /* fuel columns: mean Y over the column's fuel levels (k = 0 thread) */
         if (k == 0 && i < c.Nx - 1 && j < c.Ny - 1) {
             const int nzy = c.nzy[col];
@@ -87,16 +94,17 @@ __global__ __launch_bounds__(WF_REP_NT) void k_reports(WfConst c, WfState X, con
                                 wf_rep_reduce<0>(mw, red),      wf_rep_reduce<1>(t_lo, red),
                                 wf_rep_reduce<0>(t_hi, red),    wf_rep_reduce<1>(y_lo, red),
                                 wf_rep_reduce<0>(y_hi, red),    wf_rep_reduce<2>(ke, red),
-                                wf_rep_reduce<2>(burned, red),  wf_rep_reduce<2>(fuel_cols, red)};
+                                wf_rep_reduce<2>(burned, red),  wf_rep_reduce<2>(fuel_cols, red),
+                                wf_rep_reduce<2>(learned, red)};
     if (threadIdx.x == 0) {
 #pragma unroll
         for (int f = 0; f < WF_REP_N; f++) part[(long)blockIdx.x * WF_REP_N + f] = r[f];
     }
 }
 
-hipError_t wf_launch_reports(const WfConst &c, const WfState &X, const int *in_box, double *part, int nblocks,
-                             unsigned long long *nan_first, hipStream_t s) {
-    hipLaunchKernelGGL(k_reports, dim3(nblocks), dim3(WF_REP_NT), 0, s, c, X, in_box, part, nan_first);
+hipError_t wf_launch_reports(const WfConst &c, const WfState &X, const wf_real *mu_sgs, const int *in_box,
+                             double *part, int nblocks, unsigned long long *nan_first, hipStream_t s) {
+    hipLaunchKernelGGL(k_reports, dim3(nblocks), dim3(WF_REP_NT), 0, s, c, X, mu_sgs, in_box, part, nan_first);
     return hipGetLastError();
 }
 

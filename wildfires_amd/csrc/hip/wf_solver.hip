@@ -904,7 +904,7 @@ int reports(Solver *s, double *out, long *nan_index) {
     const WfState x = wf_view(s, s->state[s->i_yn]);
     const int nb = (int)((s->N + 255) / 256 < WF_REP_BLOCKS ? (s->N + 255) / 256 : WF_REP_BLOCKS);
     WF_HIP(hipMemsetAsync(s->rep_nan, 0xff, sizeof(unsigned long long), s->stream));
-    WF_HIP(wf_launch_reports(s->c, x, s->in_box, s->rep_part, nb, s->rep_nan, s->stream));
+    WF_HIP(wf_launch_reports(s->c, x, s->derived, s->in_box, s->rep_part, nb, s->rep_nan, s->stream));
     std::vector<double> part((size_t)nb * WF_REP_N);
     unsigned long long first = 0;
     WF_HIP(hipMemcpyAsync(part.data(), s->rep_part, sizeof(double) * part.size(), hipMemcpyDeviceToHost, s->stream));
@@ -912,7 +912,7 @@ int reports(Solver *s, double *out, long *nan_index) {
     WF_HIP(hipStreamSynchronize(s->stream));
     /* the reference's initial values (src/cu/functions.cu:86-88) */
     double mu = 0.0, mv = 0.0, mw = 0.0, t_lo = 1e9, t_hi = -1e9, y_lo = 0.0, y_hi = -1e9;
-    double ke = 0.0, burned = 0.0, cols = 0.0;
+    double ke = 0.0, burned = 0.0, cols = 0.0, learned = 0.0;
     for (int b = 0; b < nb; b++) {
         const double *r = &part[(size_t)b * WF_REP_N];
         mu = (r[0] > mu) ? r[0] : mu;
@@ -925,6 +925,7 @@ int reports(Solver *s, double *out, long *nan_index) {
         ke += r[7];
         burned += r[8];
         cols += r[9];
+        learned += r[10];
     }
     out[WF_REPORT_CFL] = P.dt * (mu / P.dx + mv / P.dy + mw / P.dz);
     out[WF_REPORT_T_MIN] = t_lo;
@@ -933,6 +934,7 @@ int reports(Solver *s, double *out, long *nan_index) {
     out[WF_REPORT_Y_MAX] = y_hi;
     out[WF_REPORT_KINETIC_ENERGY] = ke / ((double)(P.Nx - 1) * (P.Ny - 1) * (P.Nz - 2));
     out[WF_REPORT_BURNED] = (cols > 0.0) ? burned / cols : 0.0;
+    out[WF_REPORT_LEARNED] = learned / ((double)(P.Nx - 1) * (P.Ny - 1) * (P.Nz - 2));
     *nan_index = (first == ~0ull) ? -1 : (long)first;
     return WF_OK;
 }

@@ -1,11 +1,13 @@
 """Training of the learned subgrid closure on the solver's own 2x-resolution
 runs, box-filtered to the coarse grid (run on a GPU):
 
-    python -m wildfires_amd.train_closure OUT_DIR [--runs 14] [--held-out 4]
+    python -m wildfires_amd.train_closure OUT_DIR [--scales 64,128] [--runs 10] [--held-out 2]
 
-Scenario: 200 x 200 x 20 m, flat, periodic in x and y; coarse grid
-64 x 64 x 32 nodes, fine grid 127 x 127 x 63 (2x nesting of the 63 x 63 x 31
-intervals); power-law wind u_r ~ U[3, 8] m/s along x; a 2 m wide, 2 m high
+Scenario: 200 x 200 x 20 m, flat, periodic in x and y, at two scales (10
+runs each, 2 held out): coarse grid 64 x 64 x 32 nodes with fine grid
+127 x 127 x 63, and 128 x 128 x 64 with 255 x 255 x 127 (2x nesting of the
+coarse intervals), so the filter width Delta takes 1.87 m and 0.92 m;
+power-law wind u_r ~ U[3, 8] m/s along x; a 2 m wide, 2 m high
 line of hot gas (T = 1000 K) across y at x ~ U[30, 80] m over a 2 m fuel
 layer, with the combustion constants of configs.FIRE_KEYS.  At those
 constants (the largest pre-exponential factor that stays finite at these
@@ -43,8 +45,14 @@ import numpy as np
 
 from . import api, closure, configs
 
-COARSE = dict(Nx=64, Ny=64, Nz=32, x_max=200.0, y_max=200.0, z_max=20.0)
-FINE = dict(Nx=127, Ny=127, Nz=63, x_max=200.0, y_max=200.0, z_max=20.0)
+BOX = dict(x_max=200.0, y_max=200.0, z_max=20.0)
+# coarse grid and its 2x-nested fine grid per training scale (Delta = 1.87 m
+# and 0.92 m: a network trained on both takes the filter widths in between)
+GRIDS = {
+    "64": (dict(Nx=64, Ny=64, Nz=32, **BOX), dict(Nx=127, Ny=127, Nz=63, **BOX)),
+    "128": (dict(Nx=128, Ny=128, Nz=64, **BOX), dict(Nx=255, Ny=255, Nz=127, **BOX)),
+}
+COARSE, FINE = GRIDS["64"]
 SAMPLE_TIMES = (1.0, 2.0, 3.0, 4.0)
 FRONT_T = 350.0  # K: the heated front is where the hottest gas of a column crosses it
 DT_FINE = 0.005
@@ -136,10 +144,14 @@ def samples(fine_params, y_fine, dxc, dyc, dzc):
     return F, nu[sl].ravel()
 
 
-def fine_run(wd, u_r, x_ign, tag):
+def spacing(grid):
+    return tuple(grid[a + "_max"] / (grid["N" + a] - 1) for a in "xyz")
+
+
+def fine_run(wd, u_r, x_ign, tag, fine=None):
     """Snapshots of one fine FP64 GPU run at SAMPLE_TIMES: list of (t, y)."""
     n_total = int(round(SAMPLE_TIMES[-1] / DT_FINE))
-    prm = read_params(scenario_keys(FINE, u_r, x_ign, n_total, DT_FINE), wd, "fine_" + tag)
+    prm = read_params(scenario_keys(fine or FINE, u_r, x_ign, n_total, DT_FINE), wd, "fine_" + tag)
     y0, p0 = api.initial_conditions(prm)
     out = []
     with api.Solver(prm) as S:
@@ -184,12 +196,12 @@ def diagnostics(u, v, w, T, Y, x, T_inf=300.0):
             "front_x": float(np.mean(front)) if front else float("nan"), "heat_x": heat_x}
 
 
-def coarse_run(wd, u_r, x_ign, tag, block):
+def coarse_run(wd, u_r, x_ign, tag, block, coarse=None):
     """Coarse GPU run of a scenario (Smagorinsky when block is None): the
     diagnostics at SAMPLE_TIMES."""
     dt = 0.01
     n_total = int(round(SAMPLE_TIMES[-1] / dt))
-    prm = read_params(scenario_keys(COARSE, u_r, x_ign, n_total, dt), wd, "coarse_" + tag)
+    prm = read_params(scenario_keys(coarse or COARSE, u_r, x_ign, n_total, dt), wd, "coarse_" + tag)
     y0, p0 = api.initial_conditions(prm)
     out = {}
     with api.Solver(prm) as S:
@@ -229,6 +241,11 @@ def normalisation(F, q=0.1):
     into O(1e5) network inputs."""
     mean, std = F.mean(0), F.std(0)
     lo, hi = np.percentile(F, q, axis=0), np.percentile(F, 100.0 - q, axis=0)
+    # a margin, so that bounds which fall on data values (a column with a few
+    # distinct values, such as Delta over two grids) keep those values after
+    # the bounds are rounded to float32
+    pad = 1e-4 * (hi - lo) + 1e-6 * np.maximum(np.abs(lo), np.abs(hi))
+    lo, hi = lo - pad, hi + pad
     tol = np.maximum(1e-6, 1e-3 * np.abs(mean))
     deg = std < tol
     inv_std = np.where(deg, 0.0, 1.0 / np.where(deg, 1.0, std))
@@ -238,38 +255,47 @@ def normalisation(F, q=0.1):
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("out_dir")
-    ap.add_argument("--runs", type=int, default=14)
-    ap.add_argument("--held-out", type=int, default=4)
+    ap.add_argument("--scales", default="64,128", help="coarse grids (keys of GRIDS)")
+    ap.add_argument("--runs", type=int, default=10, help="runs per scale")
+    ap.add_argument("--held-out", type=int, default=2, help="held-out runs per scale")
     ap.add_argument("--rows", type=int, default=400000, help="training rows (subsampled)")
+    ap.add_argument("--rows-per-snapshot", type=int, default=60000, help="rows kept of each fine snapshot")
     ap.add_argument("--steps", type=int, default=3000)
     ap.add_argument("--seed", type=int, default=20261202)
     args = ap.parse_args(argv)
     os.makedirs(args.out_dir, exist_ok=True)
     rng = np.random.default_rng(args.seed)
     wd = tempfile.mkdtemp(prefix="wf_train_")
-    dxc = COARSE["x_max"] / (COARSE["Nx"] - 1)
-    dyc = COARSE["y_max"] / (COARSE["Ny"] - 1)
-    dzc = COARSE["z_max"] / (COARSE["Nz"] - 1)
-    runs = [(float(rng.uniform(3.0, 8.0)), float(rng.uniform(30.0, 80.0))) for _ in range(args.runs)]
+    runs = []
+    for scale in args.scales.split(","):
+        for q in range(args.runs):
+            runs.append({"scale": scale, "u_r": float(rng.uniform(3.0, 8.0)),
+                         "x_ignition": float(rng.uniform(30.0, 80.0)), "held_out": q >= args.runs - args.held_out})
     data, C_s, t0 = [], None, time.time()
     fine_diag = {}
-    xc = np.linspace(0.0, COARSE["x_max"], COARSE["Nx"])[:-1]
-    for r, (u_r, x_ign) in enumerate(runs):
-        prm, snaps = fine_run(wd, u_r, x_ign, str(r))
-        if r >= args.runs - args.held_out:
+    for r, run in enumerate(runs):
+        coarse, fine = GRIDS[run["scale"]]
+        dxc, dyc, dzc = spacing(coarse)
+        prm, snaps = fine_run(wd, run["u_r"], run["x_ignition"], str(r), fine)
+        if run["held_out"]:
+            xc = np.linspace(0.0, coarse["x_max"], coarse["Nx"])[:-1]
             fine_diag[r] = {t: filtered_diagnostics(prm, y, xc) for t, y in snaps}
         C_s = prm.p.C_s
-        parts = [samples(prm, y, dxc, dyc, dzc) for _, y in snaps]
+        parts = []
+        for _, y in snaps:
+            F, nu = samples(prm, y, dxc, dyc, dzc)
+            pick = rng.permutation(len(nu))[:args.rows_per_snapshot]
+            parts.append((F[pick], nu[pick]))
         F = np.concatenate([q[0] for q in parts])
         nu = np.concatenate([q[1] for q in parts])
-        data.append((F, nu))
-        print("run %d: u_r %.2f m/s, ignition x %.1f m: %d rows, nu mean %.3e (%.0f s)"
-              % (r, u_r, x_ign, len(nu), nu.mean(), time.time() - t0), flush=True)
-    n_train = args.runs - args.held_out
-    Ftr = np.concatenate([d[0] for d in data[:n_train]])
-    ntr = np.concatenate([d[1] for d in data[:n_train]])
-    Fho = np.concatenate([d[0] for d in data[n_train:]])
-    nho = np.concatenate([d[1] for d in data[n_train:]])
+        data.append((F, nu, run["held_out"]))
+        print("run %d (scale %s%s): u_r %.2f m/s, ignition x %.1f m: %d rows, nu mean %.3e (%.0f s)"
+              % (r, run["scale"], ", held out" if run["held_out"] else "", run["u_r"], run["x_ignition"], len(nu),
+                 nu.mean(), time.time() - t0), flush=True)
+    Ftr = np.concatenate([d[0] for d in data if not d[2]])
+    ntr = np.concatenate([d[1] for d in data if not d[2]])
+    Fho = np.concatenate([d[0] for d in data if d[2]])
+    nho = np.concatenate([d[1] for d in data if d[2]])
     # balance the rows: the flow is quiescent over most of the domain (target
     # nu ~ 0); keep the active rows (nu > 1e-6) and as many quiescent ones
     act = np.flatnonzero(ntr > 1e-6)
@@ -328,19 +354,21 @@ def main(argv=None):
     # the methods differ only where the network is used (learned rows); the
     # fallback rows are Smagorinsky in both
     rep = {
-        "runs": [{"u_r": u, "x_ignition": x} for u, x in runs], "held_out": args.held_out,
+        "runs": runs, "held_out": sum(1 for r in runs if r["held_out"]),
         "train_rows": int(len(ntr)), "held_out_rows": int(len(nh)), "loss_initial": loss0, "loss_final": loss1,
         "out_scale": out_scale, "C_s": C_s, "learned_fraction": float(L.mean()),
         "held_out_learned": score(L), "held_out_learned_active": score(a),
     }
     # a posteriori: coarse runs of the held-out scenarios against the filtered fine run
     post = []
-    for r in range(n_train, args.runs):
-        u_r, x_ign = runs[r]
-        smag = coarse_run(wd, u_r, x_ign, "s%d" % r, None)
-        clos = coarse_run(wd, u_r, x_ign, "c%d" % r, trained)
+    for r, run in enumerate(runs):
+        if not run["held_out"]:
+            continue
+        coarse = GRIDS[run["scale"]][0]
+        smag = coarse_run(wd, run["u_r"], run["x_ignition"], "s%d" % r, None, coarse)
+        clos = coarse_run(wd, run["u_r"], run["x_ignition"], "c%d" % r, trained, coarse)
         t_a, t_b = SAMPLE_TIMES[1], SAMPLE_TIMES[-1]
-        entry = {"run": r}
+        entry = {"run": r, "scale": run["scale"]}
         for name, d in (("fine_filtered", fine_diag[r]), ("smagorinsky", smag), ("closure", clos)):
             if d.get(t_b) is None or d.get(t_a) is None:
                 entry[name] = None
