@@ -163,7 +163,11 @@ __global__ __launch_bounds__(256) void k_derived(WfConst c, WfState X, WfDerived
                                      sqrt(Tx * Tx + Ty * Ty + Tz * Tz), c.delta_k[k], Ynode};
         __bf16 x[WF_CL_NIN];
         const bool in_range = wf_cl_inputs(cl.prm, f, x);
-        const float z = wf_cl_forward(cl, cl_lds[threadIdx.y], x);
+        /* a node outside the training range keeps Smagorinsky and never reads
+         * z, so a wavefront (one tile row) with no such node skips the
+         * network; its LDS block and synchronisation are its own */
+        float z = 0.0f;
+        if (__ballot(in_range) != 0ull) z = wf_cl_forward(cl, cl_lds[threadIdx.y], x);
         mu_sgs = wf_cl_mu(cl.prm, in_range, z, rho, mu_sgs);
         if (!inside) return;
     }
