@@ -351,6 +351,9 @@ def rocprof_summary(directory):
     return "\n".join(lines)
 
 
+# launches that belong to another kernel's pass: their bytes add to that
+# pass, their launches do not count
+COMPANIONS = {"stage": ("k_stage_columns", "k_stage_dup"), "derived_closure": ("k_cl_deferred", "k_cl_reset")}
 KERNEL_CLASSES = (("k_derived", "derived"), ("k_stage", "stage"), ("k_rhs_fx", "pressure_rhs_fx"),
                   ("k_fy", "pressure_fy"), ("k_thomas", "thomas"), ("k_inv_fx", "pressure_inverse_fx"),
                   ("k_project", "project"))
@@ -359,9 +362,10 @@ KERNEL_CLASSES = (("k_derived", "derived"), ("k_stage", "stage"), ("k_rhs_fx", "
 def _kernel_class(name):
     """bench kernel class of a rocprofv3 kernel name (None if not timed by
     class): the FP64 kernels (namespace wf64) by their class, the mixed form's
-    (wf32) as "mixed_<class>", the closure's derived pass as
-    "<...>derived_closure".  The top-plane transform k_rhs_fx<n, 3> runs once
-    per run, not per solve, and is left out."""
+    (wf32) as "mixed_<class>", the closure's derived pass (with its network
+    pass k_cl_deferred and queue reset k_cl_reset) as "<...>derived_closure".
+    The top-plane transform k_rhs_fx<n, 3> runs once per run, not per solve,
+    and is left out."""
     short = _short_name(name)
     prefix_ns = ""
     if short.startswith("wf32::"):
@@ -371,8 +375,10 @@ def _kernel_class(name):
     if short.startswith("k_rhs_fx") and short.replace(" ", "").endswith(",3>"):
         return None
     base = short.split("<")[0]
+    if base in COMPANIONS["derived_closure"]:
+        return prefix_ns + "derived_closure"
     for prefix, cls in KERNEL_CLASSES:
-        if base == prefix or (prefix == "k_stage" and base in ("k_stage_columns", "k_stage_dup")):
+        if base == prefix or (prefix == "k_stage" and base in COMPANIONS["stage"]):
             if cls == "derived" and short.startswith("k_derived<true>"):
                 cls = "derived_closure"
             return prefix_ns + cls
@@ -423,10 +429,10 @@ def traffic_json(pmc_dir, cal_dir, copy_bytes):
             if cls is None:
                 continue
             a = acc.setdefault(cls, {"FETCH_SIZE": [0.0, 0], "WRITE_SIZE": [0.0, 0]})
-            # k_stage_dup and k_stage_columns are companion launches of
-            # k_stage: their bytes add to its launch, they do not count
+            # companion launches: their bytes add to their pass's launch,
+            # they do not count as launches
             a[counter][0] += v * 1024.0 * scales[counter]
-            if not _short_name(name).split("::")[-1].startswith(("k_stage_columns", "k_stage_dup")):
+            if not _short_name(name).split("::")[-1].split("<")[0] in sum(COMPANIONS.values(), ()):
                 a[counter][1] += 1
     kernels = {}
     for cls, a in sorted(acc.items()):

@@ -65,6 +65,23 @@ struct WfClosure {
     const wf_bf16x8 *frag;
 };
 
+/* Nodes of a derived pass whose inputs lie in the training range, queued for
+ * the network: the node's index, its normalised inputs (two 8-vectors of
+ * bf16, the last four zero) and its density.  The queue has WF_CLQ_R
+ * regions of cap entries; wavefront g of the derived pass appends to region
+ * g % WF_CLQ_R, whose length is count[r * WF_CLQ_STRIDE] (one counter per
+ * 128-byte line, so the appends of different regions do not contend).  cap
+ * is 64 times the wavefronts a region serves, so no region overflows. */
+#define WF_CLQ_R 64
+#define WF_CLQ_STRIDE 32
+struct WfClosureQueue {
+    unsigned *count;
+    unsigned *node;
+    wf_bf16x8 *x;
+    wf_real *rho;
+    unsigned cap;
+};
+
 /* LDS per wavefront: inputs 64 x 32 bf16, hidden 16 x 64 bf16, outputs 64 floats */
 #define WF_CL_LDS_X (64 * 32)
 #define WF_CL_LDS_H (16 * 64)
@@ -120,13 +137,34 @@ __device__ __forceinline__ bool wf_cl_inputs(const float *prm, const wf_real f[W
     return in;
 }
 
+/* This lane's share of the network's parameters: its B-operand fragments of
+ * the three layers and the biases of the hidden units it produces. */
+struct WfClosureRegs {
+    wf_bf16x8 w1[4], w2[8], w3[2];
+    float b1[4], b2[4], b3;
+};
+__device__ __forceinline__ void wf_cl_load(const WfClosure &cl, WfClosureRegs &R) {
+    const int l = threadIdx.x & 63, lr = l & 15;
+    const wf_bf16x8 *W1 = cl.frag + WF_CL_FRAG_W1, *W2 = cl.frag + WF_CL_FRAG_W2, *W3 = cl.frag + WF_CL_FRAG_W3;
+#pragma unroll
+    for (int nt = 0; nt < 4; nt++) {
+        R.w1[nt] = W1[nt * 64 + l];
+        R.w2[2 * nt] = W2[(nt * 2 + 0) * 64 + l];
+        R.w2[2 * nt + 1] = W2[(nt * 2 + 1) * 64 + l];
+        R.b1[nt] = cl.prm[WF_CL_B1 + nt * 16 + lr];
+        R.b2[nt] = cl.prm[WF_CL_B2 + nt * 16 + lr];
+    }
+    R.w3[0] = W3[0 * 64 + l];
+    R.w3[1] = W3[1 * 64 + l];
+    R.b3 = cl.prm[WF_CL_B3];
+}
+
 /* Network output z for the 64 nodes of this wavefront: lane l passes its
  * node's normalised inputs and receives that node's z.  Every lane of the
  * wavefront must call it (MFMA is wave-wide); L is this wavefront's LDS. */
-__device__ __forceinline__ float wf_cl_forward(const WfClosure &cl, WfClosureLds &L, const __bf16 x[WF_CL_NIN]) {
+__device__ __forceinline__ float wf_cl_forward(const WfClosureRegs &R, WfClosureLds &L, const __bf16 x[WF_CL_NIN]) {
     const int l = threadIdx.x & 63;
     const int lr = l & 15, lg = l >> 4;
-    const float *prm = cl.prm;
     /* this lane's row of the input matrix (padded to 32 columns), as four
      * 16-byte stores */
     {
@@ -142,7 +180,6 @@ __device__ __forceinline__ float wf_cl_forward(const WfClosure &cl, WfClosureLds
         row[3] = r3;
     }
     wf_cl_wave_sync();
-    const wf_bf16x8 *W1 = cl.frag + WF_CL_FRAG_W1, *W2 = cl.frag + WF_CL_FRAG_W2, *W3 = cl.frag + WF_CL_FRAG_W3;
 #pragma unroll 1
     for (int mt = 0; mt < 4; mt++) { /* 16 nodes per tile */
         const wf_bf16x8 a1 = *(const wf_bf16x8 *)&L.x[(mt * 16 + lr) * 32 + 8 * lg];
@@ -150,15 +187,14 @@ __device__ __forceinline__ float wf_cl_forward(const WfClosure &cl, WfClosureLds
 #pragma unroll
         for (int nt = 0; nt < 4; nt++) {
             d[nt] = wf_f32x4{0.0f, 0.0f, 0.0f, 0.0f};
-            d[nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, W1[nt * 64 + l], d[nt], 0, 0, 0);
+            d[nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, R.w1[nt], d[nt], 0, 0, 0);
         }
         wf_cl_wave_sync(); /* the previous tile's reads of L.h are done */
 #pragma unroll
         for (int nt = 0; nt < 4; nt++) {
             const int n = nt * 16 + lr;
-            const float b = prm[WF_CL_B1 + n];
             const float q[4] = {d[nt][0], d[nt][1], d[nt][2], d[nt][3]};
-            wf_cl_gelu_store4(&L.h[(4 * lg) * 64 + n], q, b);
+            wf_cl_gelu_store4(&L.h[(4 * lg) * 64 + n], q, R.b1[nt]);
         }
         wf_cl_wave_sync();
         const wf_bf16x8 a2a = *(const wf_bf16x8 *)&L.h[lr * 64 + 8 * lg];
@@ -166,30 +202,34 @@ __device__ __forceinline__ float wf_cl_forward(const WfClosure &cl, WfClosureLds
 #pragma unroll
         for (int nt = 0; nt < 4; nt++) {
             d[nt] = wf_f32x4{0.0f, 0.0f, 0.0f, 0.0f};
-            d[nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a2a, W2[(nt * 2 + 0) * 64 + l], d[nt], 0, 0, 0);
-            d[nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a2b, W2[(nt * 2 + 1) * 64 + l], d[nt], 0, 0, 0);
+            d[nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a2a, R.w2[2 * nt], d[nt], 0, 0, 0);
+            d[nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a2b, R.w2[2 * nt + 1], d[nt], 0, 0, 0);
         }
         wf_cl_wave_sync();
 #pragma unroll
         for (int nt = 0; nt < 4; nt++) {
             const int n = nt * 16 + lr;
-            const float b = prm[WF_CL_B2 + n];
             const float q[4] = {d[nt][0], d[nt][1], d[nt][2], d[nt][3]};
-            wf_cl_gelu_store4(&L.h[(4 * lg) * 64 + n], q, b);
+            wf_cl_gelu_store4(&L.h[(4 * lg) * 64 + n], q, R.b2[nt]);
         }
         wf_cl_wave_sync();
         const wf_bf16x8 a3a = *(const wf_bf16x8 *)&L.h[lr * 64 + 8 * lg];
         const wf_bf16x8 a3b = *(const wf_bf16x8 *)&L.h[lr * 64 + 32 + 8 * lg];
         wf_f32x4 z = wf_f32x4{0.0f, 0.0f, 0.0f, 0.0f};
-        z = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a3a, W3[0 * 64 + l], z, 0, 0, 0);
-        z = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a3b, W3[1 * 64 + l], z, 0, 0, 0);
+        z = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a3a, R.w3[0], z, 0, 0, 0);
+        z = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a3b, R.w3[1], z, 0, 0, 0);
         if (lr == 0) {
 #pragma unroll
-            for (int r = 0; r < 4; r++) L.z[mt * 16 + 4 * lg + r] = z[r] + prm[WF_CL_B3];
+            for (int r = 0; r < 4; r++) L.z[mt * 16 + 4 * lg + r] = z[r] + R.b3;
         }
     }
     wf_cl_wave_sync();
     return L.z[l];
+}
+__device__ __forceinline__ float wf_cl_forward(const WfClosure &cl, WfClosureLds &L, const __bf16 x[WF_CL_NIN]) {
+    WfClosureRegs R;
+    wf_cl_load(cl, R);
+    return wf_cl_forward(R, L, x);
 }
 
 /* nu_t from the network output, and the node's mu_sgs under the closure:

@@ -66,7 +66,7 @@ __device__ __forceinline__ wf_real wf_heat_source(const WfConst &c, int col, int
 
 /* stage pass (wf_stage.hip) */
 hipError_t wf_launch_derived(const WfConst &c, const WfState &X, const WfDerived &D, const WfClosure *cl,
-                             hipStream_t s);
+                             const WfClosureQueue *cq, hipStream_t s);
 hipError_t wf_launch_closure_eval(const WfClosure &cl, const wf_real *feat, long n, wf_real *nu, int *learned,
                                   hipStream_t s);
 hipError_t wf_launch_stage(const WfConst &c, const WfState &X, const WfDerived &D, const WfState &Yn,

@@ -64,6 +64,10 @@ struct Solver {
     float *cl_prm;
     wf_bf16x8 *cl_frag;
     int closure_on;
+    unsigned *clq_count, *clq_node; /* queue of the nodes the network evaluates */
+    wf_bf16x8 *clq_x;
+    wf_real *clq_rho;
+    unsigned clq_cap;
     int *cut, *nzy, *radx, *rady;
     wf_real2 *twx, *twy;
     double *scratch; /* S doubles: host transfers in the reference layout */
@@ -552,7 +556,7 @@ int destroy(Solver *s) {
     if (s->stream) (void)hipStreamSynchronize(s->stream);
     for (int q = 0; q < 4; q++) (void)hipFree(s->state[q]);
     void *bufs[] = {s->zc, s->derived, s->p, s->A, s->UB, s->pmax, s->F, s->Ptop, s->z, s->cd2, s->delta_k,
-                    s->cl_prm, s->cl_frag, s->topo, s->kx2, s->ky2, s->ta,
+                    s->cl_prm, s->cl_frag, s->clq_count, s->clq_node, s->clq_x, s->clq_rho, s->topo, s->kx2, s->ky2, s->ta,
                     s->tc, s->beta, s->x, s->y, s->cut, s->nzy, s->radx, s->rady, s->twx, s->twy, s->scratch,
                     s->Tsrc, s->rho, s->ctl, s->kx2d, s->ky2d, s->tad, s->tcd, s->betad,
                     s->in_box, s->rep_part, s->rep_nan};
@@ -653,7 +657,8 @@ static int wf_stage(Solver *s, char *X, char *out, int mode, double a) {
     WfState o = wf_view(s, out);
     WfDerived D = wf_derived_view(s);
     const WfClosure cl{s->cl_prm, s->cl_frag};
-    WF_LAUNCH(s, WF_K_DERIVED, wf_launch_derived(s->c, xs, D, s->closure_on ? &cl : nullptr, s->stream));
+    const WfClosureQueue cq{s->clq_count, s->clq_node, s->clq_x, s->clq_rho, s->clq_cap};
+    WF_LAUNCH(s, WF_K_DERIVED, wf_launch_derived(s->c, xs, D, s->closure_on ? &cl : nullptr, &cq, s->stream));
     WF_LAUNCH(s, WF_K_STAGE, wf_launch_stage(s->c, xs, D, yn, acc, o, mode, a, s->closure_on != 0, s->stream));
     return WF_OK;
 }
@@ -952,14 +957,32 @@ int last_pressure(const Solver *s, double *error, int *iterations) {
 int set_closure(Solver *s, const float *params, long n) {
     if (s == NULL) return wf_fail(WF_E_ARG, "null solver");
     WF_HIP(hipStreamSynchronize(s->stream));
-    if (s->cl_prm) WF_HIP(hipFree(s->cl_prm));
-    if (s->cl_frag) WF_HIP(hipFree(s->cl_frag));
+    void *old[] = {s->cl_prm, s->cl_frag, s->clq_count, s->clq_node, s->clq_x, s->clq_rho};
+    for (size_t q = 0; q < sizeof(old) / sizeof(old[0]); q++)
+        if (old[q]) WF_HIP(hipFree(old[q]));
     s->cl_prm = NULL;
     s->cl_frag = NULL;
+    s->clq_count = s->clq_node = NULL;
+    s->clq_x = NULL;
+    s->clq_rho = NULL;
     s->closure_on = 0;
     if (params == NULL) return WF_OK;
     int rc = wf_closure_upload(params, n, &s->cl_prm, (void **)&s->cl_frag);
     if (rc) return rc;
+    /* the queue: WF_CLQ_R regions, each 64 entries per derived-pass
+     * wavefront it serves */
+    const size_t waves = (size_t)wf_tile_count(s->c) * WF_TILE_Y;
+    s->clq_cap = (unsigned)((waves + WF_CLQ_R - 1) / WF_CLQ_R * 64);
+    const size_t E = (size_t)WF_CLQ_R * s->clq_cap;
+    if (hipMalloc((void **)&s->clq_count, WF_CLQ_R * WF_CLQ_STRIDE * sizeof(unsigned)) != hipSuccess ||
+        hipMalloc((void **)&s->clq_node, E * sizeof(unsigned)) != hipSuccess ||
+        hipMalloc((void **)&s->clq_x, 2 * E * sizeof(wf_bf16x8)) != hipSuccess ||
+        hipMalloc((void **)&s->clq_rho, E * sizeof(wf_real)) != hipSuccess) {
+        (void)hipGetLastError();
+        rc = wf_fail(WF_E_DEVICE, "closure: out of device memory for the %zu-entry queue", E);
+        (void)set_closure(s, NULL, 0);
+        return rc;
+    }
     s->closure_on = 1;
     return WF_OK;
 }

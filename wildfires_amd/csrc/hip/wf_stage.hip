@@ -61,19 +61,12 @@ __device__ __forceinline__ wf_real wf_heat_q(const WfConst &c, int k, int nzy, w
 /* ---------------------------------------------------------------- pass 1 ---- */
 
 template <bool CL>
-__global__ __launch_bounds__(256) void k_derived(WfConst c, WfState X, WfDerived D, WfClosure cl) {
+__global__ __launch_bounds__(256) void k_derived(WfConst c, WfState X, WfDerived D, WfClosure cl,
+                                                 WfClosureQueue cq) {
 #pragma clang fp contract(fast)
     int i, j, k;
     wf_tile_ijk(c, &i, &j, &k);
-    /* with the closure every lane of the tile takes part in the network
-     * (nodes past the grid evaluate a clamped node and store nothing) */
-    const bool inside = (i < c.Nx) && (j < c.Ny);
-    if constexpr (CL) {
-        i = (i < c.Nx) ? i : c.Nx - 1;
-        j = (j < c.Ny) ? j : c.Ny - 1;
-    } else {
-        if (!inside) return;
-    }
+    if (i >= c.Nx || j >= c.Ny) return;
     const int Nx = c.Nx, Ny = c.Ny, Nz = c.Nz;
     const wf_real *zk = c.zc + WF_ZC * k;
     const int im1 = wf_wrap_m(i, 1, Nx), ip1 = wf_wrap_p(i, 1, Nx);
@@ -158,18 +151,33 @@ __global__ __launch_bounds__(256) void k_derived(WfConst c, WfState X, WfDerived
     const wf_real T_RHS_tmp = (dk + 12 * WR(WF_SIGMA_D) * c.delta * T2 * (Tx * Tx + Ty * Ty + Tz * Tz) +
                                (kT + 4 * WR(WF_SIGMA_D) * c.delta * T3) * lap_T) * inv_rho * c.inv_cp + q_src;
     if constexpr (CL) {
-        __shared__ WfClosureLds cl_lds[WF_TILE_Y];
+        /* a node inside the training range goes to the queue, whose network
+         * pass (k_cl_deferred) replaces the Smagorinsky value stored here;
+         * one atomic per wavefront reserves its nodes' entries */
         const wf_real f[WF_CL_NIN] = {ux, uy, uz, vx, vy, vz, wx, wy, wz,
                                      sqrt(Tx * Tx + Ty * Ty + Tz * Tz), c.delta_k[k], Ynode};
         __bf16 x[WF_CL_NIN];
         const bool in_range = wf_cl_inputs(cl.prm, f, x);
-        /* a node outside the training range keeps Smagorinsky and never reads
-         * z, so a wavefront (one tile row) with no such node skips the
-         * network; its LDS block and synchronisation are its own */
-        float z = 0.0f;
-        if (__ballot(in_range) != 0ull) z = wf_cl_forward(cl, cl_lds[threadIdx.y], x);
-        mu_sgs = wf_cl_mu(cl.prm, in_range, z, rho, mu_sgs);
-        if (!inside) return;
+        const unsigned long long m = __ballot(in_range);
+        if (m != 0ull) {
+            const int lane = (int)__lane_id(), lead = __builtin_ctzll(m);
+            const unsigned r = (blockIdx.x * WF_TILE_Y + threadIdx.y) % WF_CLQ_R;
+            unsigned base = 0u;
+            if (lane == lead) base = atomicAdd(cq.count + r * WF_CLQ_STRIDE, (unsigned)__popcll(m));
+            base = __shfl(base, lead, 64);
+            if (in_range) {
+                const unsigned e = r * cq.cap + base + (unsigned)__popcll(m & ((1ull << lane) - 1ull));
+                wf_bf16x8 a, b = {};
+#pragma unroll
+                for (int n = 0; n < 8; n++) a[n] = x[n];
+#pragma unroll
+                for (int n = 0; n < WF_CL_NIN - 8; n++) b[n] = x[8 + n];
+                cq.node[e] = q;
+                cq.x[2 * e] = a;
+                cq.x[2 * e + 1] = b;
+                cq.rho[e] = rho;
+            }
+        }
     }
     D.mu[q] = mu_sgs;
     D.s11[q] = s11;
@@ -186,17 +194,72 @@ __global__ __launch_bounds__(256) void k_derived(WfConst c, WfState X, WfDerived
     }
 }
 
+/* The network on the queued nodes, 64 to a wavefront (every lane of a
+ * wavefront in the loop takes part in the MFMA; lanes past the region's end
+ * evaluate zeros and store nothing).  Wavefront g serves region
+ * g % WF_CLQ_R, chunks g / WF_CLQ_R, + W, + 2W, ... with W the launch's
+ * wavefronts per region (the grid is a multiple of WF_CLQ_R wavefronts).
+ * The queued node's mu_sgs becomes -rho nu_t where the network gives a
+ * finite nu_t; otherwise it keeps the Smagorinsky value the derived pass
+ * stored. */
+#define WF_CL_DEFERRED_NT 256
+#define WF_CL_DEFERRED_BLOCKS 2048
+static_assert((WF_CLQ_R * 64) % WF_CL_DEFERRED_NT == 0, "deferred grid granularity");
+__global__ __launch_bounds__(WF_CL_DEFERRED_NT) void k_cl_deferred(WfClosure cl, WfClosureQueue cq, wf_real *mu) {
+    __shared__ WfClosureLds L[WF_CL_DEFERRED_NT / 64];
+    const unsigned lane = threadIdx.x & 63u, wv = threadIdx.x >> 6;
+    const unsigned g = blockIdx.x * (WF_CL_DEFERRED_NT / 64) + wv;
+    const unsigned r = g % WF_CLQ_R, W = gridDim.x * (WF_CL_DEFERRED_NT / 64) / WF_CLQ_R;
+    const unsigned n = cq.count[r * WF_CLQ_STRIDE];
+    if (n <= (g / WF_CLQ_R) * 64u) return;
+    WfClosureRegs R;
+    wf_cl_load(cl, R);
+    for (unsigned base = (g / WF_CLQ_R) * 64u; base < n; base += W * 64u) {
+        const unsigned e = r * cq.cap + base + lane;
+        const bool valid = base + lane < n;
+        wf_bf16x8 a = {}, b = {};
+        if (valid) {
+            a = cq.x[2 * e];
+            b = cq.x[2 * e + 1];
+        }
+        __bf16 x[WF_CL_NIN];
+#pragma unroll
+        for (int i = 0; i < 8; i++) x[i] = a[i];
+#pragma unroll
+        for (int i = 0; i < WF_CL_NIN - 8; i++) x[8 + i] = b[i];
+        const float z = wf_cl_forward(R, L[wv], x);
+        if (valid) {
+            const unsigned q = cq.node[e];
+            mu[q] = wf_cl_mu(cl.prm, true, z, cq.rho[e], mu[q]);
+        }
+    }
+}
+
+__global__ void k_cl_reset(unsigned *count) {
+    if (threadIdx.x < WF_CLQ_R) count[threadIdx.x * WF_CLQ_STRIDE] = 0u;
+}
+
 /* ---------------------------------------------------------------- pass 2 ---- */
 
 
-/* LDS halo tiles of one plane of a WF_TILE_X x WF_TILE_Y node tile. */
+/* Node tile of the stage pass: WF_STAGE_TX x WF_STAGE_TY columns of one plane
+ * (one thread each), and its LDS halo tiles.  32 x 8 reads fewer halo
+ * entries per node than 64 x 4 (1.69 against 2.13 for the state, 1.33
+ * against 1.55 for the derived fields) at the same 256 threads; 64 x 8 halves
+ * the workgroups per CU (LDS) and is slower. */
+#ifndef WF_STAGE_TX
+#define WF_STAGE_TX 32
+#endif
+#ifndef WF_STAGE_TY
+#define WF_STAGE_TY 8
+#endif
 #define WF_SH 2 /* state halo (upwind stencils reach i +- 2) */
 #define WF_DH 1 /* halo of the derived fields, u_tau and topography */
-#define WF_SW (WF_TILE_X + 2 * WF_SH)
-#define WF_SHH (WF_TILE_Y + 2 * WF_SH)
-#define WF_DW (WF_TILE_X + 2 * WF_DH)
-#define WF_DHH (WF_TILE_Y + 2 * WF_DH)
-#define WF_NT_STAGE (WF_TILE_X * WF_TILE_Y)
+#define WF_SW (WF_STAGE_TX + 2 * WF_SH)
+#define WF_SHH (WF_STAGE_TY + 2 * WF_SH)
+#define WF_DW (WF_STAGE_TX + 2 * WF_DH)
+#define WF_DHH (WF_STAGE_TY + 2 * WF_DH)
+#define WF_NT_STAGE (WF_STAGE_TX * WF_STAGE_TY)
 
 /* Minimum waves per SIMD that k_stage's register allocation must allow
  * (0: the compiler's choice, which is 2 at 233 VGPRs; 3 caps it at 168 and
@@ -614,7 +677,7 @@ template <bool CL>
 __global__ __launch_bounds__(WF_NT_STAGE) WF_STAGE_ATTR void k_stage(WfConst c, WfState X, WfDerived D,
                                                                      WfState Yn, WfState acc, WfState out,
                                                                      int mode, double a, int ndup) {
-    const int tx = threadIdx.x, ty = threadIdx.y, tid = tx + WF_TILE_X * ty;
+    const int tx = threadIdx.x, ty = threadIdx.y, tid = tx + WF_STAGE_TX * ty;
 #if WF_STAGE_MERGE_DUP
     if ((int)blockIdx.x < ndup) {
         const int nd = c.Nx + c.Ny - 1;
@@ -624,12 +687,12 @@ __global__ __launch_bounds__(WF_NT_STAGE) WF_STAGE_ATTR void k_stage(WfConst c, 
     }
 #endif
     __shared__ WfStageTiles L;
-    const int nty = (c.Ny + WF_TILE_Y - 1) / WF_TILE_Y;
-    const int ntx = (c.Nx + WF_TILE_X - 1) / WF_TILE_X;
+    const int nty = (c.Ny + WF_STAGE_TY - 1) / WF_STAGE_TY;
+    const int ntx = (c.Nx + WF_STAGE_TX - 1) / WF_STAGE_TX;
     int colt, k;
     /* the top plane is completed per column */
     wf_tile_decode_at((int)blockIdx.x - ndup, (int)gridDim.x - ndup, ntx * nty, c.Nz - 1, &colt, &k);
-    const int j0 = (colt % nty) * WF_TILE_Y, i0 = (colt / nty) * WF_TILE_X;
+    const int j0 = (colt % nty) * WF_STAGE_TY, i0 = (colt / nty) * WF_STAGE_TX;
     const int i = i0 + tx, j = j0 + ty;
     const bool inside = (i < c.Nx) && (j < c.Ny);
     const unsigned P = (unsigned)c.plane;
@@ -795,19 +858,32 @@ __global__ void k_to_reference(const D *src, double *dst, int Nx, int Ny, int nz
 /* ------------------------------------------------------------- launchers ---- */
 
 hipError_t wf_launch_derived(const WfConst &c, const WfState &X, const WfDerived &D, const WfClosure *cl,
-                             hipStream_t s) {
+                             const WfClosureQueue *cq, hipStream_t s) {
     dim3 blk(WF_TILE_X, WF_TILE_Y, 1), grd(wf_tile_count(c));
-    if (cl)
-        hipLaunchKernelGGL(k_derived<true>, grd, blk, 0, s, c, X, D, *cl);
-    else
-        hipLaunchKernelGGL(k_derived<false>, grd, blk, 0, s, c, X, D, WfClosure{nullptr, nullptr});
+    if (!cl) {
+        hipLaunchKernelGGL(k_derived<false>, grd, blk, 0, s, c, X, D, WfClosure{nullptr, nullptr},
+                           WfClosureQueue{nullptr, nullptr, nullptr, nullptr, 0u});
+        return hipGetLastError();
+    }
+    hipLaunchKernelGGL(k_cl_reset, dim3(1), dim3(WF_CLQ_R), 0, s, cq->count);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_derived<true>, grd, blk, 0, s, c, X, D, *cl, *cq);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    /* a quarter of the derived pass's wavefronts (a queue of up to a quarter
+     * of the nodes in one round), at most WF_CL_DEFERRED_BLOCKS workgroups,
+     * in whole multiples of WF_CLQ_R wavefronts */
+    constexpr long unit = (long)WF_CLQ_R * 64 / WF_CL_DEFERRED_NT;
+    const long want = ((long)wf_tile_count(c) * WF_TILE_Y / 4 / WF_CL_DEFERRED_NT * 64 + unit - 1) / unit * unit;
+    const unsigned nb = (unsigned)((want < unit) ? unit : (want > WF_CL_DEFERRED_BLOCKS) ? WF_CL_DEFERRED_BLOCKS : want);
+    hipLaunchKernelGGL(k_cl_deferred, dim3(nb), dim3(WF_CL_DEFERRED_NT), 0, s, *cl, *cq, D.mu);
     return hipGetLastError();
 }
 
 template <bool CL>
 static hipError_t wf_launch_stage_t(const WfConst &c, const WfState &X, const WfDerived &D, const WfState &Yn,
                                     const WfState &acc, const WfState &out, int mode, double a, hipStream_t s) {
-    const int tiles = ((c.Nx + WF_TILE_X - 1) / WF_TILE_X) * ((c.Ny + WF_TILE_Y - 1) / WF_TILE_Y) * (c.Nz - 1);
+    const int tiles = ((c.Nx + WF_STAGE_TX - 1) / WF_STAGE_TX) * ((c.Ny + WF_STAGE_TY - 1) / WF_STAGE_TY) * (c.Nz - 1);
     int ndup = 0;
     if (WF_STAGE_MERGE_DUP) {
         ndup = ((c.Nx + c.Ny - 1) * (c.Nz - 1) + WF_NT_STAGE - 1) / WF_NT_STAGE;
@@ -817,7 +893,7 @@ static hipError_t wf_launch_stage_t(const WfConst &c, const WfState &X, const Wf
         const hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
     }
-    hipLaunchKernelGGL(k_stage<CL>, dim3(ndup + tiles), dim3(WF_TILE_X, WF_TILE_Y, 1), 0, s, c, X, D, Yn, acc, out,
+    hipLaunchKernelGGL(k_stage<CL>, dim3(ndup + tiles), dim3(WF_STAGE_TX, WF_STAGE_TY, 1), 0, s, c, X, D, Yn, acc, out,
                        mode, a, ndup);
     const hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
