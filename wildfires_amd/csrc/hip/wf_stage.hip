@@ -273,12 +273,52 @@ __global__ void k_cl_reset(unsigned *count) {
 #define WF_STAGE_ATTR
 #endif
 
+/* The duplicate planes i = Nx-1 and j = Ny-1 in the tiles.  A tile position
+ * holds the periodic image in [0, n-2] of its node, which is what every
+ * neighbour read of the reference's wrap (i +- d) mod (n-1) needs; the node
+ * i = Nx-1 itself is the image of i = 0 there.  A node on the plane i = Nx-1
+ * reads its y-neighbours in its own column, i = Nx-1, so the last x tile
+ * keeps that true column in one extra LDS column (index WF_SW, WF_DW), and
+ * the last y tile the true row j = Ny-1 in one extra LDS row (index WF_SHH,
+ * WF_DHH).  Such a node reads its y- (x-) neighbours from there, everything
+ * else from the tile.  0: those nodes are evaluated by k_stage_dup from
+ * global memory instead. */
+#ifndef WF_STAGE_DUP_TILES
+#define WF_STAGE_DUP_TILES 1
+#endif
+#if WF_STAGE_DUP_TILES && WF_STAGE_MERGE_DUP
+#error "WF_STAGE_MERGE_DUP applies to the k_stage_dup form (WF_STAGE_DUP_TILES=0)"
+#endif
+#define WF_SLD (WF_SW + WF_STAGE_DUP_TILES)    /* LDS row stride, state tiles */
+#define WF_SROWS (WF_SHH + WF_STAGE_DUP_TILES) /* LDS rows, state tiles */
+#define WF_DLD (WF_DW + WF_STAGE_DUP_TILES)
+#define WF_DROWS (WF_DHH + WF_STAGE_DUP_TILES)
+
 struct WfStageTiles {
-    wf_real st[4][WF_SHH * WF_SW]; /* u, v, w, T */
-    wf_real dv[7][WF_DHH * WF_DW]; /* mu, s11, s12, s13, s22, s23, divU */
-    wf_real ut[WF_DHH * WF_DW];    /* u_tau (k = 0 plane) */
-    wf_real tp[WF_DHH * WF_DW];    /* topography */
+    wf_real st[4][WF_SROWS * WF_SLD]; /* u, v, w, T */
+    wf_real dv[7][WF_DROWS * WF_DLD]; /* mu, s11, s12, s13, s22, s23, divU */
+    wf_real ut[WF_DROWS * WF_DLD];    /* u_tau (k = 0 plane) */
+    wf_real tp[WF_DROWS * WF_DLD];    /* topography */
 };
+
+/* Source of state tile f (u, v, w, T) and of halo-1 tile f (the seven
+ * derived fields in tile order, then u_tau and topography). */
+__device__ __forceinline__ const wf_real *wf_sfield(const WfState &X, int f) {
+    return (f == 0) ? X.u : (f == 1) ? X.v : (f == 2) ? X.w : X.T;
+}
+__device__ __forceinline__ const wf_real *wf_tfield(const WfConst &c, const WfDerived &D, int f) {
+    switch (f) {
+        case 0: return D.mu;
+        case 1: return D.s11;
+        case 2: return D.s12;
+        case 3: return D.s13;
+        case 4: return D.s22;
+        case 5: return D.s23;
+        case 6: return D.divU;
+        case 7: return D.utau;
+        default: return c.topo;
+    }
+}
 
 /* Periodic neighbour of tile position g (= i0 - halo + offset) with the
  * reference's wrap (i +- d) mod (n-1): every neighbour a node 0 <= i <= n-1
@@ -288,6 +328,48 @@ __device__ __forceinline__ int wf_tile_wrap(int g, int m) {
     g = (g < 0) ? g + m : g;
     g = (g >= m) ? g - m : g;
     return (g >= m) ? 0 : g;
+}
+
+/* Entries of the extra column (the true column i = Nx-1: 4 state fields of
+ * WF_SHH rows, then 9 halo-1 fields of WF_DHH rows) and of the extra row (the
+ * true row j = Ny-1: 4 x WF_SW, then 9 x WF_DW), one per thread: the source
+ * field, the node, and the LDS place. */
+#define WF_XCOL_N (4 * WF_SHH + 9 * WF_DHH)
+#define WF_YROW_N (4 * WF_SW + 9 * WF_DW)
+static_assert(WF_XCOL_N <= WF_NT_STAGE && WF_YROW_N <= 2 * WF_NT_STAGE, "one or two extra entries per thread");
+struct WfDupEntry {
+    const wf_real *src;
+    unsigned q;
+    wf_real *dst;
+};
+__device__ __forceinline__ wf_real *wf_tile_dst(WfStageTiles &L, int f) {
+    return (f < 7) ? L.dv[f] : (f == 7) ? L.ut : L.tp;
+}
+__device__ __forceinline__ WfDupEntry wf_dup_col_entry(const WfConst &c, const WfState &X, const WfDerived &D,
+                                                      WfStageTiles &L, int j0, int k, int e) {
+    const unsigned P = (unsigned)c.plane, ix = (unsigned)(c.Nx - 1);
+    if (e < 4 * WF_SHH) {
+        const int f = e / WF_SHH, r = e - f * WF_SHH;
+        const unsigned gy = (unsigned)wf_tile_wrap(j0 - WF_SH + r, c.Ny - 1);
+        return WfDupEntry{wf_sfield(X, f), ix + (unsigned)c.Nx * gy + P * (unsigned)k, &L.st[f][r * WF_SLD + WF_SW]};
+    }
+    e -= 4 * WF_SHH;
+    const int f = e / WF_DHH, r = e - f * WF_DHH;
+    const unsigned cl = ix + (unsigned)c.Nx * (unsigned)wf_tile_wrap(j0 - WF_DH + r, c.Ny - 1);
+    return WfDupEntry{wf_tfield(c, D, f), (f < 7) ? cl + P * (unsigned)k : cl, wf_tile_dst(L, f) + r * WF_DLD + WF_DW};
+}
+__device__ __forceinline__ WfDupEntry wf_dup_row_entry(const WfConst &c, const WfState &X, const WfDerived &D,
+                                                      WfStageTiles &L, int i0, int k, int e) {
+    const unsigned P = (unsigned)c.plane, row = (unsigned)c.Nx * (unsigned)(c.Ny - 1);
+    if (e < 4 * WF_SW) {
+        const int f = e / WF_SW, cx = e - f * WF_SW;
+        const unsigned gx = (unsigned)wf_tile_wrap(i0 - WF_SH + cx, c.Nx - 1);
+        return WfDupEntry{wf_sfield(X, f), row + gx + P * (unsigned)k, &L.st[f][WF_SHH * WF_SLD + cx]};
+    }
+    e -= 4 * WF_SW;
+    const int f = e / WF_DW, cx = e - f * WF_DW;
+    const unsigned cl = row + (unsigned)wf_tile_wrap(i0 - WF_DH + cx, c.Nx - 1);
+    return WfDupEntry{wf_tfield(c, D, f), (f < 7) ? cl + P * (unsigned)k : cl, wf_tile_dst(L, f) + WF_DHH * WF_DLD + cx};
 }
 
 /* Loads of one halo tile (width W, height HH, halo H) of plane k of `src`
@@ -309,14 +391,15 @@ __device__ __forceinline__ void wf_tile_load(wf_real *v, const wf_real *src, con
     }
 }
 
-template <int W, int HH>
+template <int W, int HH, int LD>
 __device__ __forceinline__ void wf_tile_store(wf_real *dst, const wf_real *v, int tid) {
     constexpr int NE = W * HH;
     constexpr int PER = (NE + WF_NT_STAGE - 1) / WF_NT_STAGE;
 #pragma unroll
     for (int p = 0; p < PER; p++) {
         const int e = tid + p * WF_NT_STAGE;
-        if (e < NE) dst[e] = v[p];
+        const int r = e / W;
+        if (e < NE) dst[(LD == W) ? e : e + r * (LD - W)] = v[p];
     }
 }
 
@@ -342,10 +425,17 @@ enum { WF_D_MU = 0, WF_D_S11, WF_D_S12, WF_D_S13, WF_D_S22, WF_D_S23, WF_D_DIVU,
 struct WfTileAcc {
     const WfStageTiles &L;
     const WfColumnVals &V;
-    int sc, dc;
-    __device__ __forceinline__ wf_real st(int f, int di, int dj) const { return L.st[f][sc + di + dj * WF_SW]; }
+    /* LDS index of the node for its x-neighbour reads (sx, dx) and its
+     * y-neighbour reads (sy, dy): the node's own tile position, except on a
+     * duplicate plane (the extra row for x, the extra column for y) */
+    int sx, sy, dx, dy;
+    __device__ __forceinline__ wf_real st(int f, int di, int dj) const {
+        return (dj == 0) ? L.st[f][sx + di] : L.st[f][sy + dj * WF_SLD];
+    }
     __device__ __forceinline__ wf_real sz(int f, int dk) const { return V.s[f][2 + dk]; }
-    __device__ __forceinline__ wf_real dv(int f, int di, int dj) const { return L.dv[f][dc + di + dj * WF_DW]; }
+    __device__ __forceinline__ wf_real dv(int f, int di, int dj) const {
+        return (dj == 0) ? L.dv[f][dx + di] : L.dv[f][dy + dj * WF_DLD];
+    }
     __device__ __forceinline__ wf_real dz(int f, int dk) const {
         switch (f) {
             case WF_D_MU: return V.mu[1 + dk];
@@ -359,10 +449,10 @@ struct WfTileAcc {
         }
     }
     __device__ __forceinline__ wf_real ut(int di, int dj) const {
-        return (di == 0 && dj == 0) ? V.utau : L.ut[dc + di + dj * WF_DW];
+        return (di == 0 && dj == 0) ? V.utau : (dj == 0) ? L.ut[dx + di] : L.ut[dy + dj * WF_DLD];
     }
     __device__ __forceinline__ wf_real tp(int di, int dj) const {
-        return (di == 0 && dj == 0) ? V.topo : L.tp[dc + di + dj * WF_DW];
+        return (di == 0 && dj == 0) ? V.topo : (dj == 0) ? L.tp[dx + di] : L.tp[dy + dj * WF_DLD];
     }
     __device__ __forceinline__ wf_real Y() const { return V.Y; }
 };
@@ -405,8 +495,24 @@ struct WfGlobalAcc {
         return wf_ld(dfield(f), (unsigned)wf_q(c, ni(di), nj(dj), k));
     }
     __device__ __forceinline__ wf_real dz(int f, int dk) const { return wf_ld(dfield(f), (unsigned)wf_q(c, i, j, k + dk)); }
-    __device__ __forceinline__ wf_real ut(int di, int dj) const { return D.utau[ni(di) + c.Nx * nj(dj)]; }
-    __device__ __forceinline__ wf_real tp(int di, int dj) const { return c.topo[ni(di) + c.Nx * nj(dj)]; }
+    /* u_tau and topography of the column and its four neighbours (0, +x,
+     * -x, +y, -y), read by load() with the other operands: read where the
+     * damping uses them, each pair waited for in turn behind the branch of
+     * the previous one */
+    wf_real utn[5], tpn[5];
+    __device__ __forceinline__ void load() {
+        const int cols[5] = {i + c.Nx * j, ni(1) + c.Nx * j, ni(-1) + c.Nx * j, i + c.Nx * nj(1), i + c.Nx * nj(-1)};
+#pragma unroll
+        for (int n = 0; n < 5; n++) {
+            utn[n] = D.utau[cols[n]];
+            tpn[n] = c.topo[cols[n]];
+        }
+    }
+    __device__ __forceinline__ static int nb(int di, int dj) {
+        return (di > 0) ? 1 : (di < 0) ? 2 : (dj > 0) ? 3 : (dj < 0) ? 4 : 0;
+    }
+    __device__ __forceinline__ wf_real ut(int di, int dj) const { return utn[nb(di, dj)]; }
+    __device__ __forceinline__ wf_real tp(int di, int dj) const { return tpn[nb(di, dj)]; }
     __device__ __forceinline__ wf_real Y() const {
         return (k < c.Nzm) ? (wf_real)wf_ld(X.Y, (unsigned)wf_q(c, i, j, k)) : WR(0.0);
     }
@@ -654,7 +760,8 @@ __device__ __forceinline__ void wf_stage_dup_node(const WfConst &c, const WfStat
         r[3] = c.T_dead;
         r[4] = (k < nzy) ? (wf_real)c.Y_dead : WR(0.0);
     } else {
-        const WfGlobalAcc acc_g{c, X, D, i, j, k};
+        WfGlobalAcc acc_g{c, X, D, i, j, k};
+        acc_g.load();
         wf_rhs_node<CL>(c, acc_g, k, nzy, r);
         wf_keep_raw(c, D, col, k, cut, r);
     }
@@ -698,6 +805,30 @@ __global__ __launch_bounds__(WF_NT_STAGE) WF_STAGE_ATTR void k_stage(WfConst c, 
     const unsigned P = (unsigned)c.plane;
     const int col = inside ? i + c.Nx * j : 0;
     const unsigned q = (unsigned)col + P * (unsigned)k;
+#if WF_STAGE_DUP_TILES
+    /* the last x tile's extra column and the last y tile's extra row: their
+     * loads go out first, to share the wait of the tile loads */
+    const bool dup_x = (i0 + WF_STAGE_TX >= c.Nx) && tid < WF_XCOL_N;
+    const bool dup_y0 = (j0 + WF_STAGE_TY >= c.Ny) && tid < WF_YROW_N;
+    const bool dup_y1 = (j0 + WF_STAGE_TY >= c.Ny) && tid + WF_NT_STAGE < WF_YROW_N;
+    wf_real dup_x_v = WR(0.0), dup_y_v[2] = {WR(0.0), WR(0.0)};
+    wf_real *dup_x_dst = nullptr, *dup_y0_dst = nullptr, *dup_y1_dst = nullptr;
+    if (dup_x) {
+        const WfDupEntry d = wf_dup_col_entry(c, X, D, L, j0, k, tid);
+        dup_x_dst = d.dst;
+        dup_x_v = wf_ld(d.src, d.q);
+    }
+    if (dup_y0) {
+        const WfDupEntry d = wf_dup_row_entry(c, X, D, L, i0, k, tid);
+        dup_y0_dst = d.dst;
+        dup_y_v[0] = wf_ld(d.src, d.q);
+    }
+    if (dup_y1) {
+        const WfDupEntry d = wf_dup_row_entry(c, X, D, L, i0, k, tid + WF_NT_STAGE);
+        dup_y1_dst = d.dst;
+        dup_y_v[1] = wf_ld(d.src, d.q);
+    }
+#endif
     /* halo tiles of plane k */
     {
         constexpr int PS = (WF_SW * WF_SHH + WF_NT_STAGE - 1) / WF_NT_STAGE;
@@ -713,11 +844,16 @@ __global__ __launch_bounds__(WF_NT_STAGE) WF_STAGE_ATTR void k_stage(WfConst c, 
         wf_tile_load<WF_DW, WF_DHH, WF_DH>(vu, D.utau, c, i0, j0, 0, tid);
         wf_tile_load<WF_DW, WF_DHH, WF_DH>(vt, c.topo, c, i0, j0, 0, tid);
 #pragma unroll
-        for (int f = 0; f < 4; f++) wf_tile_store<WF_SW, WF_SHH>(L.st[f], vs[f], tid);
+        for (int f = 0; f < 4; f++) wf_tile_store<WF_SW, WF_SHH, WF_SLD>(L.st[f], vs[f], tid);
 #pragma unroll
-        for (int f = 0; f < 7; f++) wf_tile_store<WF_DW, WF_DHH>(L.dv[f], vd[f], tid);
-        wf_tile_store<WF_DW, WF_DHH>(L.ut, vu, tid);
-        wf_tile_store<WF_DW, WF_DHH>(L.tp, vt, tid);
+        for (int f = 0; f < 7; f++) wf_tile_store<WF_DW, WF_DHH, WF_DLD>(L.dv[f], vd[f], tid);
+        wf_tile_store<WF_DW, WF_DHH, WF_DLD>(L.ut, vu, tid);
+        wf_tile_store<WF_DW, WF_DHH, WF_DLD>(L.tp, vt, tid);
+#if WF_STAGE_DUP_TILES
+        if (dup_x) *dup_x_dst = dup_x_v;
+        if (dup_y0) *dup_y0_dst = dup_y_v[0];
+        if (dup_y1) *dup_y1_dst = dup_y_v[1];
+#endif
     }
     /* the node's own column (its own values, also on the duplicate planes) */
     WfColumnVals V;
@@ -760,10 +896,14 @@ __global__ __launch_bounds__(WF_NT_STAGE) WF_STAGE_ATTR void k_stage(WfConst c, 
     WfRkIn rk;
     wf_rk_load(c, mode, q, (unsigned)col, k, Yn, acc, rk);
     __syncthreads();
+#if WF_STAGE_DUP_TILES
+    if (!inside || k == cut) return;
+#else
     /* the duplicate planes i = Nx-1, j = Ny-1 read their x- (y-) neighbours
      * in their own row (column), which the tiles hold as row (column) 0:
      * k_stage_dup evaluates them */
     if (!inside || k == cut || i == c.Nx - 1 || j == c.Ny - 1) return;
+#endif
     wf_real r[5];
     if (k < cut) {
         r[0] = c.u_dead;
@@ -772,7 +912,10 @@ __global__ __launch_bounds__(WF_NT_STAGE) WF_STAGE_ATTR void k_stage(WfConst c, 
         r[3] = c.T_dead;
         r[4] = (k < nzy) ? (wf_real)c.Y_dead : WR(0.0);
     } else {
-        const WfTileAcc acc_t{L, V, (ty + WF_SH) * WF_SW + (tx + WF_SH), (ty + WF_DH) * WF_DW + (tx + WF_DH)};
+        const int s0 = (ty + WF_SH) * WF_SLD + (tx + WF_SH), d0 = (ty + WF_DH) * WF_DLD + (tx + WF_DH);
+        const bool xd = WF_STAGE_DUP_TILES && i == c.Nx - 1, yd = WF_STAGE_DUP_TILES && j == c.Ny - 1;
+        const WfTileAcc acc_t{L, V, yd ? WF_SHH * WF_SLD + (tx + WF_SH) : s0, xd ? (ty + WF_SH) * WF_SLD + WF_SW : s0,
+                              yd ? WF_DHH * WF_DLD + (tx + WF_DH) : d0, xd ? (ty + WF_DH) * WF_DLD + WF_DW : d0};
         wf_rhs_node<CL>(c, acc_t, k, nzy, r);
         wf_keep_raw(c, D, col, k, cut, r);
     }
@@ -885,7 +1028,9 @@ static hipError_t wf_launch_stage_t(const WfConst &c, const WfState &X, const Wf
                                     const WfState &acc, const WfState &out, int mode, double a, hipStream_t s) {
     const int tiles = ((c.Nx + WF_STAGE_TX - 1) / WF_STAGE_TX) * ((c.Ny + WF_STAGE_TY - 1) / WF_STAGE_TY) * (c.Nz - 1);
     int ndup = 0;
-    if (WF_STAGE_MERGE_DUP) {
+    if (WF_STAGE_DUP_TILES) {
+        /* the tiles evaluate the duplicate planes */
+    } else if (WF_STAGE_MERGE_DUP) {
         ndup = ((c.Nx + c.Ny - 1) * (c.Nz - 1) + WF_NT_STAGE - 1) / WF_NT_STAGE;
     } else {
         hipLaunchKernelGGL(k_stage_dup<CL>, dim3((c.Nx + c.Ny - 1 + 255) / 256, c.Nz - 1), dim3(256), 0, s, c, X, D,
