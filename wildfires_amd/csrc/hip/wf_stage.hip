@@ -301,25 +301,6 @@ struct WfStageTiles {
     wf_real tp[WF_DROWS * WF_DLD];    /* topography */
 };
 
-/* Source of state tile f (u, v, w, T) and of halo-1 tile f (the seven
- * derived fields in tile order, then u_tau and topography). */
-__device__ __forceinline__ const wf_real *wf_sfield(const WfState &X, int f) {
-    return (f == 0) ? X.u : (f == 1) ? X.v : (f == 2) ? X.w : X.T;
-}
-__device__ __forceinline__ const wf_real *wf_tfield(const WfConst &c, const WfDerived &D, int f) {
-    switch (f) {
-        case 0: return D.mu;
-        case 1: return D.s11;
-        case 2: return D.s12;
-        case 3: return D.s13;
-        case 4: return D.s22;
-        case 5: return D.s23;
-        case 6: return D.divU;
-        case 7: return D.utau;
-        default: return c.topo;
-    }
-}
-
 /* Periodic neighbour of tile position g (= i0 - halo + offset) with the
  * reference's wrap (i +- d) mod (n-1): every neighbour a node 0 <= i <= n-1
  * reads (|d| <= 2) lands on its image in [0, n-2].  Positions beyond the last
@@ -330,76 +311,35 @@ __device__ __forceinline__ int wf_tile_wrap(int g, int m) {
     return (g >= m) ? 0 : g;
 }
 
-/* Entries of the extra column (the true column i = Nx-1: 4 state fields of
- * WF_SHH rows, then 9 halo-1 fields of WF_DHH rows) and of the extra row (the
- * true row j = Ny-1: 4 x WF_SW, then 9 x WF_DW), one per thread: the source
- * field, the node, and the LDS place. */
-#define WF_XCOL_N (4 * WF_SHH + 9 * WF_DHH)
-#define WF_YROW_N (4 * WF_SW + 9 * WF_DW)
-static_assert(WF_XCOL_N <= WF_NT_STAGE && WF_YROW_N <= 2 * WF_NT_STAGE, "one or two extra entries per thread");
-struct WfDupEntry {
-    const wf_real *src;
-    unsigned q;
-    wf_real *dst;
-};
-__device__ __forceinline__ wf_real *wf_tile_dst(WfStageTiles &L, int f) {
-    return (f < 7) ? L.dv[f] : (f == 7) ? L.ut : L.tp;
-}
-__device__ __forceinline__ WfDupEntry wf_dup_col_entry(const WfConst &c, const WfState &X, const WfDerived &D,
-                                                      WfStageTiles &L, int j0, int k, int e) {
-    const unsigned P = (unsigned)c.plane, ix = (unsigned)(c.Nx - 1);
-    if (e < 4 * WF_SHH) {
-        const int f = e / WF_SHH, r = e - f * WF_SHH;
-        const unsigned gy = (unsigned)wf_tile_wrap(j0 - WF_SH + r, c.Ny - 1);
-        return WfDupEntry{wf_sfield(X, f), ix + (unsigned)c.Nx * gy + P * (unsigned)k, &L.st[f][r * WF_SLD + WF_SW]};
-    }
-    e -= 4 * WF_SHH;
-    const int f = e / WF_DHH, r = e - f * WF_DHH;
-    const unsigned cl = ix + (unsigned)c.Nx * (unsigned)wf_tile_wrap(j0 - WF_DH + r, c.Ny - 1);
-    return WfDupEntry{wf_tfield(c, D, f), (f < 7) ? cl + P * (unsigned)k : cl, wf_tile_dst(L, f) + r * WF_DLD + WF_DW};
-}
-__device__ __forceinline__ WfDupEntry wf_dup_row_entry(const WfConst &c, const WfState &X, const WfDerived &D,
-                                                      WfStageTiles &L, int i0, int k, int e) {
-    const unsigned P = (unsigned)c.plane, row = (unsigned)c.Nx * (unsigned)(c.Ny - 1);
-    if (e < 4 * WF_SW) {
-        const int f = e / WF_SW, cx = e - f * WF_SW;
-        const unsigned gx = (unsigned)wf_tile_wrap(i0 - WF_SH + cx, c.Nx - 1);
-        return WfDupEntry{wf_sfield(X, f), row + gx + P * (unsigned)k, &L.st[f][WF_SHH * WF_SLD + cx]};
-    }
-    e -= 4 * WF_SW;
-    const int f = e / WF_DW, cx = e - f * WF_DW;
-    const unsigned cl = row + (unsigned)wf_tile_wrap(i0 - WF_DH + cx, c.Nx - 1);
-    return WfDupEntry{wf_tfield(c, D, f), (f < 7) ? cl + P * (unsigned)k : cl, wf_tile_dst(L, f) + WF_DHH * WF_DLD + cx};
-}
-
 /* Loads of one halo tile (width W, height HH, halo H) of plane k of `src`
- * into registers; all lanes load (out-of-tile entries read node 0) so the
- * loads issue without branches. */
-template <int W, int HH, int H>
+ * into registers, as ROWS x LD entries: with the duplicate planes in the
+ * tiles (LD = W + 1, ROWS = HH + 1) column W holds the true column i = Nx-1
+ * and row HH the true row j = Ny-1 (every tile loads them; only the last x
+ * and y tiles read them).  All lanes load (out-of-tile entries read node 0)
+ * so the loads issue without branches. */
+template <int W, int HH, int H, int LD, int ROWS>
 __device__ __forceinline__ void wf_tile_load(wf_real *v, const wf_real *src, const WfConst &c, int i0, int j0, int k,
                                              int tid) {
-    constexpr int NE = W * HH;
+    constexpr int NE = LD * ROWS;
     constexpr int PER = (NE + WF_NT_STAGE - 1) / WF_NT_STAGE;
 #pragma unroll
     for (int p = 0; p < PER; p++) {
         const int e = tid + p * WF_NT_STAGE;
-        const int r = e / W, cx = e - r * W;
-        const int gx = wf_tile_wrap(i0 - H + cx, c.Nx - 1);
-        const int gy = wf_tile_wrap(j0 - H + r, c.Ny - 1);
+        const int r = e / LD, cx = e - r * LD;
+        const int gx = (LD > W && cx == W) ? c.Nx - 1 : wf_tile_wrap(i0 - H + cx, c.Nx - 1);
+        const int gy = (ROWS > HH && r == HH) ? c.Ny - 1 : wf_tile_wrap(j0 - H + r, c.Ny - 1);
         const unsigned q = (e < NE) ? (unsigned)(gx + c.Nx * (gy + c.Ny * k)) : 0u;
         v[p] = wf_ld(src, q);
     }
 }
 
-template <int W, int HH, int LD>
+template <int NE>
 __device__ __forceinline__ void wf_tile_store(wf_real *dst, const wf_real *v, int tid) {
-    constexpr int NE = W * HH;
     constexpr int PER = (NE + WF_NT_STAGE - 1) / WF_NT_STAGE;
 #pragma unroll
     for (int p = 0; p < PER; p++) {
         const int e = tid + p * WF_NT_STAGE;
-        const int r = e / W;
-        if (e < NE) dst[(LD == W) ? e : e + r * (LD - W)] = v[p];
+        if (e < NE) dst[e] = v[p];
     }
 }
 
@@ -805,55 +745,28 @@ __global__ __launch_bounds__(WF_NT_STAGE) WF_STAGE_ATTR void k_stage(WfConst c, 
     const unsigned P = (unsigned)c.plane;
     const int col = inside ? i + c.Nx * j : 0;
     const unsigned q = (unsigned)col + P * (unsigned)k;
-#if WF_STAGE_DUP_TILES
-    /* the last x tile's extra column and the last y tile's extra row: their
-     * loads go out first, to share the wait of the tile loads */
-    const bool dup_x = (i0 + WF_STAGE_TX >= c.Nx) && tid < WF_XCOL_N;
-    const bool dup_y0 = (j0 + WF_STAGE_TY >= c.Ny) && tid < WF_YROW_N;
-    const bool dup_y1 = (j0 + WF_STAGE_TY >= c.Ny) && tid + WF_NT_STAGE < WF_YROW_N;
-    wf_real dup_x_v = WR(0.0), dup_y_v[2] = {WR(0.0), WR(0.0)};
-    wf_real *dup_x_dst = nullptr, *dup_y0_dst = nullptr, *dup_y1_dst = nullptr;
-    if (dup_x) {
-        const WfDupEntry d = wf_dup_col_entry(c, X, D, L, j0, k, tid);
-        dup_x_dst = d.dst;
-        dup_x_v = wf_ld(d.src, d.q);
-    }
-    if (dup_y0) {
-        const WfDupEntry d = wf_dup_row_entry(c, X, D, L, i0, k, tid);
-        dup_y0_dst = d.dst;
-        dup_y_v[0] = wf_ld(d.src, d.q);
-    }
-    if (dup_y1) {
-        const WfDupEntry d = wf_dup_row_entry(c, X, D, L, i0, k, tid + WF_NT_STAGE);
-        dup_y1_dst = d.dst;
-        dup_y_v[1] = wf_ld(d.src, d.q);
-    }
-#endif
     /* halo tiles of plane k */
     {
-        constexpr int PS = (WF_SW * WF_SHH + WF_NT_STAGE - 1) / WF_NT_STAGE;
-        constexpr int PD = (WF_DW * WF_DHH + WF_NT_STAGE - 1) / WF_NT_STAGE;
+        constexpr int NS = WF_SLD * WF_SROWS, ND = WF_DLD * WF_DROWS;
+        constexpr int PS = (NS + WF_NT_STAGE - 1) / WF_NT_STAGE;
+        constexpr int PD = (ND + WF_NT_STAGE - 1) / WF_NT_STAGE;
         wf_real vs[4][PS], vd[7][PD], vu[PD], vt[PD];
-        wf_tile_load<WF_SW, WF_SHH, WF_SH>(vs[0], X.u, c, i0, j0, k, tid);
-        wf_tile_load<WF_SW, WF_SHH, WF_SH>(vs[1], X.v, c, i0, j0, k, tid);
-        wf_tile_load<WF_SW, WF_SHH, WF_SH>(vs[2], X.w, c, i0, j0, k, tid);
-        wf_tile_load<WF_SW, WF_SHH, WF_SH>(vs[3], X.T, c, i0, j0, k, tid);
+        wf_tile_load<WF_SW, WF_SHH, WF_SH, WF_SLD, WF_SROWS>(vs[0], X.u, c, i0, j0, k, tid);
+        wf_tile_load<WF_SW, WF_SHH, WF_SH, WF_SLD, WF_SROWS>(vs[1], X.v, c, i0, j0, k, tid);
+        wf_tile_load<WF_SW, WF_SHH, WF_SH, WF_SLD, WF_SROWS>(vs[2], X.w, c, i0, j0, k, tid);
+        wf_tile_load<WF_SW, WF_SHH, WF_SH, WF_SLD, WF_SROWS>(vs[3], X.T, c, i0, j0, k, tid);
         const wf_real *dsrc[7] = {D.mu, D.s11, D.s12, D.s13, D.s22, D.s23, D.divU};
 #pragma unroll
-        for (int f = 0; f < 7; f++) wf_tile_load<WF_DW, WF_DHH, WF_DH>(vd[f], dsrc[f], c, i0, j0, k, tid);
-        wf_tile_load<WF_DW, WF_DHH, WF_DH>(vu, D.utau, c, i0, j0, 0, tid);
-        wf_tile_load<WF_DW, WF_DHH, WF_DH>(vt, c.topo, c, i0, j0, 0, tid);
+        for (int f = 0; f < 7; f++)
+            wf_tile_load<WF_DW, WF_DHH, WF_DH, WF_DLD, WF_DROWS>(vd[f], dsrc[f], c, i0, j0, k, tid);
+        wf_tile_load<WF_DW, WF_DHH, WF_DH, WF_DLD, WF_DROWS>(vu, D.utau, c, i0, j0, 0, tid);
+        wf_tile_load<WF_DW, WF_DHH, WF_DH, WF_DLD, WF_DROWS>(vt, c.topo, c, i0, j0, 0, tid);
 #pragma unroll
-        for (int f = 0; f < 4; f++) wf_tile_store<WF_SW, WF_SHH, WF_SLD>(L.st[f], vs[f], tid);
+        for (int f = 0; f < 4; f++) wf_tile_store<NS>(L.st[f], vs[f], tid);
 #pragma unroll
-        for (int f = 0; f < 7; f++) wf_tile_store<WF_DW, WF_DHH, WF_DLD>(L.dv[f], vd[f], tid);
-        wf_tile_store<WF_DW, WF_DHH, WF_DLD>(L.ut, vu, tid);
-        wf_tile_store<WF_DW, WF_DHH, WF_DLD>(L.tp, vt, tid);
-#if WF_STAGE_DUP_TILES
-        if (dup_x) *dup_x_dst = dup_x_v;
-        if (dup_y0) *dup_y0_dst = dup_y_v[0];
-        if (dup_y1) *dup_y1_dst = dup_y_v[1];
-#endif
+        for (int f = 0; f < 7; f++) wf_tile_store<ND>(L.dv[f], vd[f], tid);
+        wf_tile_store<ND>(L.ut, vu, tid);
+        wf_tile_store<ND>(L.tp, vt, tid);
     }
     /* the node's own column (its own values, also on the duplicate planes) */
     WfColumnVals V;
