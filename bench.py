@@ -353,7 +353,7 @@ def rocprof_summary(directory):
 
 # launches that belong to another kernel's pass: their bytes add to that
 # pass, their launches do not count
-COMPANIONS = {"stage": ("k_stage_columns", "k_stage_dup"), "derived_closure": ("k_cl_deferred", "k_cl_reset")}
+COMPANIONS = {"stage": ("k_stage_columns", "k_stage_dup"), "derived_closure": ("k_cl_deferred",)}
 KERNEL_CLASSES = (("k_derived", "derived"), ("k_stage", "stage"), ("k_rhs_fx", "pressure_rhs_fx"),
                   ("k_fy", "pressure_fy"), ("k_thomas", "thomas"), ("k_inv_fx", "pressure_inverse_fx"),
                   ("k_project", "project"))
@@ -363,7 +363,7 @@ def _kernel_class(name):
     """bench kernel class of a rocprofv3 kernel name (None if not timed by
     class): the FP64 kernels (namespace wf64) by their class, the mixed form's
     (wf32) as "mixed_<class>", the closure's derived pass (with its network
-    pass k_cl_deferred and queue reset k_cl_reset) as "<...>derived_closure".
+    pass k_cl_deferred) as "<...>derived_closure".
     The top-plane transform k_rhs_fx<n, 3> runs once per run, not per solve,
     and is left out."""
     short = _short_name(name)

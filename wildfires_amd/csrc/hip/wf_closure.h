@@ -76,6 +76,7 @@ struct WfClosure {
 #define WF_CLQ_STRIDE 32
 struct WfClosureQueue {
     unsigned *count;
+    unsigned *next; /* the counters of the following derived pass, zeroed by the network pass */
     unsigned *node;
     wf_bf16x8 *x;
     wf_real *rho;

@@ -68,6 +68,7 @@ struct Solver {
     wf_bf16x8 *clq_x;
     wf_real *clq_rho;
     unsigned clq_cap;
+    int clq_parity; /* which of the two counter sets the next derived pass appends to */
     int *cut, *nzy, *radx, *rady;
     wf_real2 *twx, *twy;
     double *scratch; /* S doubles: host transfers in the reference layout */
@@ -657,8 +658,11 @@ static int wf_stage(Solver *s, char *X, char *out, int mode, double a) {
     WfState o = wf_view(s, out);
     WfDerived D = wf_derived_view(s);
     const WfClosure cl{s->cl_prm, s->cl_frag};
-    const WfClosureQueue cq{s->clq_count, s->clq_node, s->clq_x, s->clq_rho, s->clq_cap};
+    unsigned *cnt = s->clq_count + (s->clq_parity ? WF_CLQ_R * WF_CLQ_STRIDE : 0);
+    unsigned *nxt = s->clq_count + (s->clq_parity ? 0 : WF_CLQ_R * WF_CLQ_STRIDE);
+    const WfClosureQueue cq{cnt, nxt, s->clq_node, s->clq_x, s->clq_rho, s->clq_cap};
     WF_LAUNCH(s, WF_K_DERIVED, wf_launch_derived(s->c, xs, D, s->closure_on ? &cl : nullptr, &cq, s->stream));
+    s->clq_parity ^= s->closure_on ? 1 : 0;
     WF_LAUNCH(s, WF_K_STAGE, wf_launch_stage(s->c, xs, D, yn, acc, o, mode, a, s->closure_on != 0, s->stream));
     return WF_OK;
 }
@@ -974,7 +978,8 @@ int set_closure(Solver *s, const float *params, long n) {
     const size_t waves = (size_t)wf_tile_count(s->c) * WF_TILE_Y;
     s->clq_cap = (unsigned)((waves + WF_CLQ_R - 1) / WF_CLQ_R * 64);
     const size_t E = (size_t)WF_CLQ_R * s->clq_cap;
-    if (hipMalloc((void **)&s->clq_count, WF_CLQ_R * WF_CLQ_STRIDE * sizeof(unsigned)) != hipSuccess ||
+    if (hipMalloc((void **)&s->clq_count, 2 * WF_CLQ_R * WF_CLQ_STRIDE * sizeof(unsigned)) != hipSuccess ||
+        hipMemsetAsync(s->clq_count, 0, 2 * WF_CLQ_R * WF_CLQ_STRIDE * sizeof(unsigned), s->stream) != hipSuccess ||
         hipMalloc((void **)&s->clq_node, E * sizeof(unsigned)) != hipSuccess ||
         hipMalloc((void **)&s->clq_x, 2 * E * sizeof(wf_bf16x8)) != hipSuccess ||
         hipMalloc((void **)&s->clq_rho, E * sizeof(wf_real)) != hipSuccess) {
@@ -983,6 +988,7 @@ int set_closure(Solver *s, const float *params, long n) {
         (void)set_closure(s, NULL, 0);
         return rc;
     }
+    s->clq_parity = 0;
     s->closure_on = 1;
     return WF_OK;
 }

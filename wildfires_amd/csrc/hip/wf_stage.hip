@@ -207,6 +207,9 @@ __global__ __launch_bounds__(256) void k_derived(WfConst c, WfState X, WfDerived
 static_assert((WF_CLQ_R * 64) % WF_CL_DEFERRED_NT == 0, "deferred grid granularity");
 __global__ __launch_bounds__(WF_CL_DEFERRED_NT) void k_cl_deferred(WfClosure cl, WfClosureQueue cq, wf_real *mu) {
     __shared__ WfClosureLds L[WF_CL_DEFERRED_NT / 64];
+    /* the following derived pass appends to the other counter set: its last
+     * reader was the previous network pass, which has completed */
+    if (blockIdx.x == 0 && threadIdx.x < WF_CLQ_R) cq.next[threadIdx.x * WF_CLQ_STRIDE] = 0u;
     const unsigned lane = threadIdx.x & 63u, wv = threadIdx.x >> 6;
     const unsigned g = blockIdx.x * (WF_CL_DEFERRED_NT / 64) + wv;
     const unsigned r = g % WF_CLQ_R, W = gridDim.x * (WF_CL_DEFERRED_NT / 64) / WF_CLQ_R;
@@ -235,9 +238,6 @@ __global__ __launch_bounds__(WF_CL_DEFERRED_NT) void k_cl_deferred(WfClosure cl,
     }
 }
 
-__global__ void k_cl_reset(unsigned *count) {
-    if (threadIdx.x < WF_CLQ_R) count[threadIdx.x * WF_CLQ_STRIDE] = 0u;
-}
 
 /* ---------------------------------------------------------------- pass 2 ---- */
 
@@ -918,14 +918,12 @@ hipError_t wf_launch_derived(const WfConst &c, const WfState &X, const WfDerived
     dim3 blk(WF_TILE_X, WF_TILE_Y, 1), grd(wf_tile_count(c));
     if (!cl) {
         hipLaunchKernelGGL(k_derived<false>, grd, blk, 0, s, c, X, D, WfClosure{nullptr, nullptr},
-                           WfClosureQueue{nullptr, nullptr, nullptr, nullptr, 0u});
+                           WfClosureQueue{nullptr, nullptr, nullptr, nullptr, nullptr, 0u});
         return hipGetLastError();
     }
-    hipLaunchKernelGGL(k_cl_reset, dim3(1), dim3(WF_CLQ_R), 0, s, cq->count);
+    hipLaunchKernelGGL(k_derived<true>, grd, blk, 0, s, c, X, D, *cl, *cq);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_derived<true>, grd, blk, 0, s, c, X, D, *cl, *cq);
-    if ((e = hipGetLastError()) != hipSuccess) return e;
     /* a quarter of the derived pass's wavefronts (a queue of up to a quarter
      * of the nodes in one round), at most WF_CL_DEFERRED_BLOCKS workgroups,
      * in whole multiples of WF_CLQ_R wavefronts */
