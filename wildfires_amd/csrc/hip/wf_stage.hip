@@ -8,9 +8,10 @@
  * Pass 2 (k_stage) evaluates the full RHS of u, v, w, T, Y at a node
  * (src/cu/pde.cu:139-787 plus src/cu/turbulence.cu:255-454) and writes the RK
  * combination (src/cu/utils.cu:14-21, src/cu/solver.cu:14-39) instead of
- * storing the stage derivative; k_stage_dup does the same for the duplicate
- * periodic planes, and k_stage_columns completes the cut and top nodes of
- * every column (src/cu/pde.cu:14-99).
+ * storing the stage derivative, the duplicate periodic planes included (or,
+ * built with WF_STAGE_DUP_TILES=0, k_stage_dup does those), and
+ * k_stage_columns completes the cut and top nodes of every column
+ * (src/cu/pde.cu:14-99).
  *
  * Arithmetic: the reference's difference formulas with their constant
  * divisors (2 dx, dx^2, the non-uniform z weights, c_p, Pr, ...) taken as
@@ -436,9 +437,9 @@ struct WfGlobalAcc {
     }
     __device__ __forceinline__ wf_real dz(int f, int dk) const { return wf_ld(dfield(f), (unsigned)wf_q(c, i, j, k + dk)); }
     /* u_tau and topography of the column and its four neighbours (0, +x,
-     * -x, +y, -y), read by load() with the other operands: read where the
-     * damping uses them, each pair waited for in turn behind the branch of
-     * the previous one */
+     * -x, +y, -y), read by load() before the RHS so that they go out with
+     * the other operands (read where the damping uses them, each pair
+     * would wait behind the previous damping factor's branch) */
     wf_real utn[5], tpn[5];
     __device__ __forceinline__ void load() {
         const int cols[5] = {i + c.Nx * j, ni(1) + c.Nx * j, ni(-1) + c.Nx * j, i + c.Nx * nj(1), i + c.Nx * nj(-1)};
@@ -712,11 +713,11 @@ __device__ __forceinline__ void wf_stage_dup_node(const WfConst &c, const WfStat
  * dead-node values, fluid nodes cut < k <= Nz-2 their RHS; the raw RHS of the
  * nodes cut+1, cut+2, Nz-2 and Nz-3 is also kept per column for the cut and
  * top nodes, which k_stage_columns completes (src/cu/pde.cu:14-99).
- * With WF_STAGE_MERGE_DUP the first ndup blocks evaluate the duplicate-plane
- * nodes one per thread (their tiles cannot hold them, see below), so their
- * longer-latency waves overlap the tile blocks; that path needs more registers
- * than the tiles, so by default k_stage_dup evaluates them in a launch of its
- * own. */
+ * The duplicate-plane nodes are evaluated by their tiles (WF_STAGE_DUP_TILES).
+ * Built without that, they are evaluated one per thread from global memory:
+ * by k_stage_dup in a launch of its own or, with WF_STAGE_MERGE_DUP, by the
+ * first ndup blocks of this launch, so that their longer-latency waves
+ * overlap the tile blocks (that path needs more registers than the tiles). */
 #ifndef WF_STAGE_MERGE_DUP
 #define WF_STAGE_MERGE_DUP 0
 #endif
